@@ -1,0 +1,212 @@
+/*
+ * mi3hip.h -- C ABI of libmi3hip.so, the MI355X (gfx950) engine behind the
+ * Mi3 inference round.
+ *
+ * The reference reaches its device code through pyopencl kernel enqueues
+ * issued by the per-device wrapper class MCMCGPU (mi3gpu/mcmcGPU.py:110-1018)
+ * on named device buffers.  This header is that boundary as a C ABI: plain
+ * pointers, sizes and NUL-terminated buffer names, no OpenCL, HIP or torch
+ * types.  Each entry point cites the reference call it replaces.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative MI3_E* code; the
+ *    message of the last failure on a context is mi3_last_error(ctx).
+ *  - A context owns one device, one in-order HIP stream and a table of named
+ *    device buffers; all compute entry points are asynchronous on that
+ *    stream (the reference's event chaining, mcmcGPU.py:211-222, becomes
+ *    stream order).  Contexts are not thread-safe.
+ *  - Buffer names follow the reference (mcmcGPU.py:173-184, 271-330):
+ *    "J", "dJ", "bi", "bi target", "bicount", "seq main", "seq large",
+ *    "E main", "E large", "E tmp", "weights", "weights large",
+ *    "weightstats", "minout", "cprob", ...  The device layout of "seq *"
+ *    buffers is site-major: L rows of `ld` bytes, row i holding residue i of
+ *    every sequence (host code converts from/to the reference's (N, L) uint8).
+ */
+#ifndef MI3HIP_H
+#define MI3HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI3_OK 0
+#define MI3_EINVAL (-1)   /* bad argument: ValueError in the reference wrapper */
+#define MI3_ENOBUF (-2)   /* unknown buffer name */
+#define MI3_ESTATE (-3)   /* component not initialised / already initialised */
+#define MI3_EHIP (-4)     /* HIP runtime error */
+#define MI3_ENOMEM (-5)   /* allocation failed */
+#define MI3_EUNSUP (-6)   /* size outside what this build supports (e.g. q > 25) */
+
+typedef struct mi3_ctx mi3_ctx;
+
+/* Library version string. */
+const char *mi3_version(void);
+
+/* Number of visible HIP devices (printGPUs, mcmcGPU.py:58-63). */
+int mi3_device_count(int *count);
+
+/* Device name and properties as text (printDevice, mcmcGPU.py:45-56). */
+int mi3_device_info(int device, char *buf, size_t buflen);
+
+/* Create a context for an L-site, q-letter Potts model with `nwalkers`
+ * walkers (multiple of 512) on `device`.  Replaces MCMCGPU.__init__
+ * (mcmcGPU.py:111-193) together with setup_GPU_context / initGPU
+ * (mcmcGPU.py:1038-1119): no program is compiled at run time, L and q are
+ * kernel arguments.  `seed` keys every counter-based random stream.
+ * `profile` != 0 records one event pair per launch (the reference's
+ * PROFILING_ENABLE queue, mcmcGPU.py:153-157). */
+int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t seed, int profile,
+               mi3_ctx **out);
+
+/* Release every device buffer and the stream. */
+int mi3_destroy(mi3_ctx *ctx);
+
+/* Message of the last failure on `ctx` (or of the last mi3_create failure
+ * when ctx is NULL). */
+const char *mi3_last_error(mi3_ctx *ctx);
+
+/* ---- named buffers ------------------------------------------------------ */
+
+/* Allocate a named device buffer of `nbytes` (zero-filled).  Replaces
+ * MCMCGPU._setupBuffer (mcmcGPU.py:239-259). */
+int mi3_buf_alloc(mi3_ctx *ctx, const char *name, size_t nbytes);
+
+/* 1 if `name` exists, else 0. */
+int mi3_buf_exists(mi3_ctx *ctx, const char *name);
+
+/* Size in bytes of a named buffer. */
+int mi3_buf_size(mi3_ctx *ctx, const char *name, size_t *nbytes);
+
+/* Copy host memory into a buffer at byte `offset`, ordered on the stream;
+ * returns when the host memory may be reused.  Replaces MCMCGPU.setBuf with a
+ * host array (mcmcGPU.py:869-907). */
+int mi3_buf_write(mi3_ctx *ctx, const char *name, const void *src, size_t nbytes, size_t offset);
+
+/* Device-to-device copy (setBuf with a device buffer, mcmcGPU.py:873-879). */
+int mi3_buf_copy(mi3_ctx *ctx, const char *dst, const char *src, size_t nbytes);
+
+/* Enqueue a read of `nbytes` at `offset` into library-owned pinned staging;
+ * returns a ticket.  The data is a snapshot taken in stream order, as
+ * getBuf's enqueue_copy (mcmcGPU.py:845-867). */
+int mi3_buf_read_async(mi3_ctx *ctx, const char *name, size_t nbytes, size_t offset, int *ticket);
+
+/* Wait for a ticket and copy its data to `dst` (FutureBuf.read,
+ * mcmcGPU.py:95-108).  The ticket is released. */
+int mi3_ticket_wait(mi3_ctx *ctx, int ticket, void *dst, size_t nbytes);
+
+/* Fill a buffer with a 32-bit pattern (fillBuf, mcmcGPU.py:909-917).  Uses a
+ * fill kernel, so it is safe inside captured graphs. */
+int mi3_buf_fill_u32(mi3_ctx *ctx, const char *name, uint32_t pattern);
+
+/* Wait for all work on the context's stream (MCMCGPU.wait, mcmcGPU.py:1016). */
+int mi3_synchronize(mi3_ctx *ctx);
+
+/* ---- sequences ---------------------------------------------------------- */
+
+/* Fill every sequence of a seq buffer with one sequence of L residues
+ * (fillSeqs, mcmcGPU.py:927-932). */
+int mi3_fill_seqs(mi3_ctx *ctx, const char *seqbuf, const uint8_t *seq);
+
+/* ---- MCMC ---------------------------------------------------------------- */
+
+/* Set nsteps per runMCMC call and the walker-stream offset (initMCMC,
+ * mcmcGPU.py:271-283, and _initMCMC_RNG 410-445).  `rng_offset` /
+ * `rng_span` select a disjoint walker-index range per device, as the
+ * reference's per-GPU stream spans (node_manager.py:73-85). */
+int mi3_init_mcmc(mi3_ctx *ctx, uint32_t nsteps, uint64_t rng_offset, uint64_t rng_span);
+
+/* nsteps Metropolis site updates of every walker of "seq main" under the
+ * couplings "J" at inverse temperature beta.  Replaces MCMCGPU.runMCMC
+ * (mcmcGPU.py:460-482) and the metropolis kernel (mcmc.cl:415-458). */
+int mi3_run_mcmc(mi3_ctx *ctx, float beta);
+
+/* Number of MCMC steps executed so far on this context (the global step
+ * counter of the random streams). */
+int mi3_mcmc_step_count(mi3_ctx *ctx, uint64_t *steps);
+
+/* Independent-site reseed of a seq buffer from "cprob" (gen_indep,
+ * mcmcGPU.py:380-391, mcmc.cl:170-244). */
+int mi3_gen_indep(mi3_ctx *ctx, const char *seqbuf);
+
+/* ---- energies and marginals ---------------------------------------------- */
+
+/* Ebuf[n] = sum_{i<j} Jbuf[pair(i,j), q*s_i + s_j] for the first nseq
+ * sequences of seqbuf.  Replaces calcEnergies (mcmcGPU.py:536-553) and
+ * getEnergies (mcmc.cl:249-345). */
+int mi3_calc_energies(mi3_ctx *ctx, const char *seqbuf, const char *Jbuf, const char *Ebuf,
+                      uint32_t nseq);
+
+/* "bicount"[pair][a*q+b] = #{n < nseq : s_i = a, s_j = b}, exact uint32.
+ * Replaces calcBicounts (mcmcGPU.py:501-518) and countBivariate
+ * (mcmc.cl:464-516). */
+int mi3_calc_bicounts(mi3_ctx *ctx, const char *seqbuf, uint32_t nseq);
+
+/* "bi" = float("bicount") / float(nseq) (bicounts_to_bimarg,
+ * mcmcGPU.py:520-534, mcmc.cl:518-528). */
+int mi3_bicounts_to_bimarg(mi3_ctx *ctx, uint32_t nseq);
+
+/* "bi"[pair][a*q+b] = sum_n wbuf[n] [s_i=a][s_j=b] over the first nseq
+ * sequences (weightedMarg, mcmcGPU.py:634-661, mcmc.cl:646-718). */
+int mi3_weighted_marg(mi3_ctx *ctx, const char *seqbuf, const char *wbuf, uint32_t nseq);
+
+/* Divide each pair block of "bi" by its sum (renormalize_bimarg,
+ * mcmcGPU.py:663-668; merge_bimarg on one device, node_manager.py:283-287). */
+int mi3_renormalize_bimarg(mi3_ctx *ctx);
+
+/* ---- quasi-Newton step ---------------------------------------------------- */
+
+/* "minout"[0] = min(0, min buf[0..n)) (min_buf, mcmcGPU.py:555-567;
+ * minFloats, mcmc.cl:558-584). */
+int mi3_min_buf(mi3_ctx *ctx, const char *buf, uint32_t n);
+
+/* "weightstats" = (sum w, sum w^2) over wbuf[0..n) (weight_statistics,
+ * mcmcGPU.py:569-589; weight_stats, mcmc.cl:586-622). */
+int mi3_weight_stats(mi3_ctx *ctx, const char *wbuf, uint32_t n);
+
+/* wbuf = exp(-Ebuf + ref) (dE_to_weights, mcmcGPU.py:591-610,
+ * mcmc.cl:624-633). */
+int mi3_dE_to_weights(mi3_ctx *ctx, const char *Ebuf, const char *wbuf, uint32_t n, float ref);
+
+/* wbuf = exp((beta-1)*(Ebuf - ref)) (fixed_beta_weights, mcmcGPU.py:612-631,
+ * mcmc.cl:635-644). */
+int mi3_fixed_beta_weights(mi3_ctx *ctx, const char *Ebuf, const char *wbuf, uint32_t n,
+                           float ref, float beta);
+
+/* dst += src over the first n floats (addFloatBuf, mcmcGPU.py:670-685). */
+int mi3_add_float_bufs(mi3_ctx *ctx, const char *dst, const char *src, uint32_t n);
+
+/* Jbuf <- Jbuf - gamma*("bi target" - "bi")/("bi" + pc) (updateJ,
+ * mcmcGPU.py:704-718; updatedJ, mcmc.cl:829-843). */
+int mi3_update_J(mi3_ctx *ctx, float gamma, float pc, const char *Jbuf);
+
+/* Zero-gauge L1 / L2 step on "dJ" given "J" and "bi" (reg_l1z / reg_l2z,
+ * mcmcGPU.py:720-745; mcmc.cl:845-902). */
+int mi3_reg_l1z(mi3_ctx *ctx, float gamma, float pc, float lJ);
+int mi3_reg_l2z(mi3_ctx *ctx, float gamma, float pc, float lJ);
+
+/* updateJ on "dJ" fused with the following reg_l1z (reg = 1), reg_l2z
+ * (reg = 2) or nothing (reg = 0) in one pass over the pair blocks: the
+ * sequence NewtonSteps.iterNewton issues every iteration
+ * (NewtonSteps.py:218-220). */
+int mi3_update_J_reg(mi3_ctx *ctx, float gamma, float pc, int reg, float lJ);
+
+/* ---- profiling ------------------------------------------------------------ */
+
+/* Append "EVT name start_ns end_ns bytes" lines for every completed launch to
+ * `path` (logProfile, mcmcGPU.py:224-237; the format read by
+ * utils/analyze_gpu_profiling.py:4-36).  Returns the number written. */
+int mi3_profile_flush(mi3_ctx *ctx, const char *path);
+
+/* Average duration in ms of the launches of kernel `name` recorded since the
+ * last reset, and their count. */
+int mi3_profile_kernel_ms(mi3_ctx *ctx, const char *name, double *avg_ms, int *count);
+int mi3_profile_reset(mi3_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MI3HIP_H */

@@ -1,0 +1,262 @@
+// Quasi-Newton (Zwanzig reweighting) kernels of one inference round.
+#include "common.h"
+
+namespace mi3 {
+
+// bimarg = float(count) / float(nseq), correctly rounded (mcmc.cl:518-528).
+__global__ void k_bicounts_to_bimarg(const uint32_t* __restrict__ c, float* __restrict__ bi,
+                                     uint32_t nseq, uint64_t n) {
+    uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) bi[idx] = __fdiv_rn((float)c[idx], (float)nseq);
+}
+
+// Per q x q block: coupling update (updatedJ, mcmc.cl:829-843)
+//   dJ <- dJ - gamma*(f_target - f)/(f + pc)
+// optionally followed by the zero-gauge L1/L2 step (reg_l1z / reg_l2z,
+// mcmc.cl:845-902, with zeroGauge 764-822):
+//   J0 = zerogauge(J + dJ); R = -lam*sign(J0)*gamma/(f+pc)   (l1)
+//                            R = -lam*J0*gamma/(f+pc)        (l2)
+//   dJ <- (sign(J0) != sign(J0+R)) ? dJ - J0 : dJ + R
+// One wave per pair block; `mode` 0 = update only, 1 = reg only (l1),
+// 2 = reg only (l2), 3 = update + l1, 4 = update + l2.
+__device__ __forceinline__ float fsign(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+
+__global__ __launch_bounds__(256) void k_update_reg(const float* __restrict__ bt,
+                                                    const float* __restrict__ bi,
+                                                    const float* __restrict__ J,
+                                                    float* __restrict__ dJ, uint32_t nP,
+                                                    uint32_t q, float gamma, float pc, float lam,
+                                                    int mode) {
+    __shared__ float sJ[4][32 * 32];
+    __shared__ float sh[4][2][32];
+    const uint32_t wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * 4 + wave;
+    const uint32_t qq = q * q;
+    const bool active = p < nP;
+    const bool upd = (mode == 0) || (mode >= 3);
+    const int reg = (mode == 1 || mode == 3) ? 1 : ((mode == 2 || mode == 4) ? 2 : 0);
+    float dj[16];  // q*q <= 1024 -> at most 16 per lane
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        uint32_t e = l + 64 * r;
+        dj[r] = 0.f;
+        if (active && e < qq) {
+            size_t o = (size_t)p * qq + e;
+            float d = dJ[o];
+            if (upd) {
+                float f = bi[o];
+                d = __fsub_rn(d, __fdiv_rn(__fmul_rn(gamma, __fsub_rn(bt[o], f)), __fadd_rn(f, pc)));
+            }
+            dj[r] = d;
+            if (reg) sJ[wave][e] = __fadd_rn(J[o], d);
+        }
+    }
+    if (reg) {
+        __syncthreads();
+        // row means (lanes 0..q-1) and column means (lanes 32..32+q-1)
+        if (active) {
+            if (l < q) {
+                float s = 0.f;
+                for (uint32_t b = 0; b < q; b++) s += sJ[wave][l * q + b];
+                sh[wave][0][l] = s / q;
+            } else if (l >= 32 && l - 32 < q) {
+                uint32_t b = l - 32;
+                float s = 0.f;
+                for (uint32_t a = 0; a < q; a++) s += sJ[wave][a * q + b];
+                sh[wave][1][b] = s / q;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            float mean = 0.f;
+            for (uint32_t a = 0; a < q; a++) mean += sh[wave][0][a];
+            mean /= q;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                uint32_t e = l + 64 * r;
+                if (e < qq) {
+                    uint32_t a = e / q, b = e - a * q;
+                    float hi = sh[wave][0][a] - mean, hj = sh[wave][1][b] - mean;
+                    float J0 = sJ[wave][e] - (hi + hj) - mean;
+                    size_t o = (size_t)p * qq + e;
+                    float R = (reg == 1) ? -lam * fsign(J0) * gamma / (bi[o] + pc)
+                                         : -lam * J0 * gamma / (bi[o] + pc);
+                    dj[r] = (fsign(J0) != fsign(J0 + R)) ? dj[r] - J0 : dj[r] + R;
+                }
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            uint32_t e = l + 64 * r;
+            if (e < qq) dJ[(size_t)p * qq + e] = dj[r];
+        }
+    }
+}
+
+// Divide every pair block by its sum (renormalize_bimarg, mcmc.cl:729-759).
+__global__ __launch_bounds__(256) void k_renormalize(float* __restrict__ bi, uint32_t nP,
+                                                     uint32_t qq) {
+    const uint32_t wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * 4 + wave;
+    if (p >= nP) return;
+    float* b = bi + (size_t)p * qq;
+    float s = 0.f;
+    for (uint32_t e = l; e < qq; e += 64) s += b[e];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    for (uint32_t e = l; e < qq; e += 64) b[e] = b[e] / s;
+}
+
+// Two-pass deterministic reductions over n floats.  Pass 1 writes one
+// partial per block: min (starting from 0, as minFloats, mcmc.cl:558-584),
+// sum w, sum w^2 (weight_stats, mcmc.cl:586-622) in float64.
+__global__ __launch_bounds__(256) void k_reduce_partial(const float* __restrict__ x, uint32_t n,
+                                                        int op, double* __restrict__ part) {
+    __shared__ double s1[256], s2[256];
+    uint32_t tid = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (uint32_t i = blockIdx.x * 256 + tid; i < n; i += gridDim.x * 256) {
+        double v = x[i];
+        if (op == 0)
+            a = fmin(a, v);
+        else {
+            a += v;
+            b += v * v;
+        }
+    }
+    s1[tid] = a;
+    s2[tid] = b;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (tid < off) {
+            if (op == 0)
+                s1[tid] = fmin(s1[tid], s1[tid + off]);
+            else {
+                s1[tid] += s1[tid + off];
+                s2[tid] += s2[tid + off];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        part[2 * blockIdx.x] = s1[0];
+        part[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(const double* __restrict__ part,
+                                                      uint32_t nb, int op,
+                                                      float* __restrict__ out) {
+    __shared__ double s1[256], s2[256];
+    uint32_t tid = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (uint32_t i = tid; i < nb; i += 256) {
+        if (op == 0)
+            a = fmin(a, part[2 * i]);
+        else {
+            a += part[2 * i];
+            b += part[2 * i + 1];
+        }
+    }
+    s1[tid] = a;
+    s2[tid] = b;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (tid < off) {
+            if (op == 0)
+                s1[tid] = fmin(s1[tid], s1[tid + off]);
+            else {
+                s1[tid] += s1[tid + off];
+                s2[tid] += s2[tid + off];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[0] = (float)s1[0];
+        if (op != 0) out[1] = (float)s2[0];
+    }
+}
+
+// w = exp(-dE + ref) (dE_to_weights, mcmc.cl:624-633).
+__global__ void k_dE_to_weights(const float* __restrict__ dE, float* __restrict__ w, float ref,
+                                uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = expf(-dE[i] + ref);
+}
+
+// w = exp((beta-1)*(E - ref)) (fixed_beta_weights, mcmc.cl:635-644).
+__global__ void k_fixed_beta_weights(const float* __restrict__ E, float* __restrict__ w,
+                                     float ref, float bm1, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = expf(bm1 * (E[i] - ref));
+}
+
+// dst += src (addFloatBufs, mcmc.cl:720-727).
+__global__ void k_add_floats(float* __restrict__ dst, const float* __restrict__ src, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] += src[i];
+}
+
+// Weighted pair histograms F_{im}(a,b) = sum_n w_n [s_ni = a][s_nm = b]
+// (weightedMarg, mcmc.cl:646-718), unnormalised.  One work-group per
+// (row i, chunk of 32 partner sites m > i): the 32 pair histograms live in
+// LDS as [a*q+b][32] (lane j of each half-wave owns pair (i, 32c+j), so the
+// LDS adds of a half-wave hit 32 distinct banks); every half-wave streams
+// 16 sequences per iteration and the block is written once at the end.
+__global__ __launch_bounds__(256) void k_weighted_marg(const uint8_t* __restrict__ seq,
+                                                       uint32_t ldseq, uint32_t nseq,
+                                                       const float* __restrict__ w, uint32_t L,
+                                                       uint32_t q, uint32_t nchunk,
+                                                       const uint32_t* __restrict__ units,
+                                                       float* __restrict__ out) {
+    extern __shared__ float hist[];
+    const uint32_t unit = units[blockIdx.x];
+    const uint32_t i = unit >> 8, c = unit & 0xff;
+    const uint32_t qq = q * q;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t e = tid; e < qq * CH; e += 256) hist[e] = 0.f;
+    __syncthreads();
+    const uint32_t lane = tid & 63, j = lane & 31, hw = tid >> 5;  // 8 half-waves
+    const uint32_t m = c * CH + j;
+    const bool valid = (m > i) && (m < L);
+    const uint32_t mrow = valid ? m : i;
+    const uint8_t* rowi = seq + (size_t)i * ldseq;
+    const uint8_t* rowm = seq + (size_t)mrow * ldseq;
+    const uint32_t qch = q * CH;
+    for (uint32_t n0 = hw * 16; n0 < nseq; n0 += 8 * 16) {
+        if (n0 + 16 <= nseq) {
+            uint4 si4 = *reinterpret_cast<const uint4*>(rowi + n0);
+            uint4 sm4 = *reinterpret_cast<const uint4*>(rowm + n0);
+            const float4* w4 = reinterpret_cast<const float4*>(w + n0);
+            float4 wv[4] = {w4[0], w4[1], w4[2], w4[3]};
+            uint32_t si[4] = {si4.x, si4.y, si4.z, si4.w};
+            uint32_t sm[4] = {sm4.x, sm4.y, sm4.z, sm4.w};
+            float ww[16] = {wv[0].x, wv[0].y, wv[0].z, wv[0].w, wv[1].x, wv[1].y, wv[1].z, wv[1].w,
+                            wv[2].x, wv[2].y, wv[2].z, wv[2].w, wv[3].x, wv[3].y, wv[3].z, wv[3].w};
+            if (valid) {
+#pragma unroll
+                for (int e = 0; e < 16; e++) {
+                    uint32_t a = (si[e >> 2] >> (8 * (e & 3))) & 0xffu;
+                    uint32_t b = (sm[e >> 2] >> (8 * (e & 3))) & 0xffu;
+                    atomicAdd(&hist[a * qch + b * CH + j], ww[e]);
+                }
+            }
+        } else if (valid) {
+            for (uint32_t n = n0; n < nseq && n < n0 + 16; n++) {
+                uint32_t a = rowi[n], b = rowm[n];
+                atomicAdd(&hist[a * qch + b * CH + j], w[n]);
+            }
+        }
+    }
+    __syncthreads();
+    // write the 32 pair blocks (packed order), skipping invalid pairs
+    for (uint32_t e = tid; e < qq * CH; e += 256) {
+        uint32_t jj = e & 31, ac = e >> 5;
+        uint32_t mm = c * CH + jj;
+        if (mm > i && mm < L) out[(size_t)pair_index(i, mm, L) * qq + ac] = hist[e];
+    }
+}
+
+}  // namespace mi3

@@ -1,0 +1,53 @@
+// Host-side launchers shared between translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mi3 {
+
+struct SweepArgs {
+    const float* Ju;  // chunked layout, chunk stride chunk_floats(q)
+    uint8_t* seq;
+    const uint32_t* pos;
+    uint32_t ld, nwalk, L, q, nchunk, nsteps, k0, k1, walker_base;
+    uint64_t step0;
+    float nbeta;
+};
+
+struct EnergyArgs {
+    const float* Ju;
+    const uint8_t* seq;
+    float* E;
+    uint32_t ld, nseq, L, q, nchunk;
+};
+
+// Walkers per work-group of the sweep kernel is 256 (L <= 256) or 128; the
+// walker count must be a multiple of 256.
+constexpr uint32_t SWEEP_WALKERS_PER_BLOCK = 256;
+
+// Largest q / L the register-resident kernels are instantiated for.
+constexpr uint32_t MAX_Q = 25;
+constexpr uint32_t MAX_L = 512;
+
+// Threads per work-group of the sweep / energy kernels.
+constexpr uint32_t POTTS_THREADS = 512;
+
+// float4 loads per thread to stage one chunk of q*q*32 floats; the chunked
+// coupling layout pads every chunk to POTTS_THREADS * pf_class(q) float4 so
+// that staging needs no bounds checks.
+inline uint32_t pf_class(uint32_t q) {
+    uint32_t need = (q * q * 32 / 4 + POTTS_THREADS - 1) / POTTS_THREADS;
+    if (need <= 1) return 1;
+    if (need <= 4) return 4;
+    if (need <= 7) return 7;
+    return 10;
+}
+
+// Floats per (padded) chunk.
+inline uint32_t chunk_floats(uint32_t q) { return pf_class(q) * POTTS_THREADS * 4; }
+
+// Returns a hipError_t; grid/LDS sizing done inside.
+hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
+hipError_t launch_energies(const EnergyArgs& a, hipStream_t st);
+
+}  // namespace mi3

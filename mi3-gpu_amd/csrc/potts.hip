@@ -1,0 +1,86 @@
+// Instantiation and launch of the register-resident sweep / energy kernels.
+#include "k_potts.hip"
+#include "launch.h"
+
+namespace mi3 {
+
+namespace {
+constexpr int NWAVES = 8;
+constexpr int NT = 64 * NWAVES;
+// walker slots per half-wave: fewer for long sequences so state stays in registers
+template <int KW>
+constexpr int slots() { return KW <= 2 ? 16 : 8; }
+
+static_assert(NT == POTTS_THREADS, "thread count");
+
+template <typename F>
+hipError_t set_lds(F kernel, size_t bytes) {
+    static_assert(sizeof(F) > 0, "");
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <int KW, int PF>
+hipError_t sweep_kw(const SweepArgs& a, hipStream_t st) {
+    constexpr uint32_t W = 2 * slots<KW>() * NWAVES;
+    auto kern = k_sweep<KW, slots<KW>(), NWAVES, PF>;
+    size_t lds = 2ull * chunk_floats(a.q) * sizeof(float);
+    hipError_t e = set_lds(kern, lds);
+    if (e != hipSuccess) return e;
+    if (a.nwalk % W) return hipErrorInvalidValue;
+    dim3 grid(a.nwalk / W);
+    hipLaunchKernelGGL(kern, grid, dim3(NT), lds, st, a.Ju, a.seq, a.pos, a.ld, a.L, a.q,
+                       a.nchunk, a.nsteps, a.k0, a.k1, a.step0, a.walker_base, a.nbeta);
+    return hipGetLastError();
+}
+
+template <int KW, int PF>
+hipError_t energies_kw(const EnergyArgs& a, hipStream_t st) {
+    constexpr uint32_t W = 2 * slots<KW>() * NWAVES;
+    auto kern = k_energies<KW, slots<KW>(), NWAVES, PF>;
+    size_t lds = 2ull * chunk_floats(a.q) * sizeof(float);
+    hipError_t e = set_lds(kern, lds);
+    if (e != hipSuccess) return e;
+    dim3 grid((a.nseq + W - 1) / W);
+    hipLaunchKernelGGL(kern, grid, dim3(NT), lds, st, a.Ju, a.seq, a.E, a.ld, a.nseq, a.L, a.q,
+                       a.nchunk);
+    return hipGetLastError();
+}
+
+template <template <int, int> class Disp, typename A>
+hipError_t dispatch(const A& a, hipStream_t st, uint32_t nchunk, uint32_t q) {
+    uint32_t kw = (nchunk + 3) / 4;
+    int pf = (int)pf_class(q);
+#define MI3_CASE(KW_, PF_) \
+    if (kw == KW_ && pf == PF_) return Disp<KW_, PF_>::run(a, st);
+    MI3_CASE(1, 1) MI3_CASE(1, 4) MI3_CASE(1, 7) MI3_CASE(1, 10)
+    MI3_CASE(2, 1) MI3_CASE(2, 4) MI3_CASE(2, 7) MI3_CASE(2, 10)
+    MI3_CASE(3, 1) MI3_CASE(3, 4) MI3_CASE(3, 7) MI3_CASE(3, 10)
+    MI3_CASE(4, 1) MI3_CASE(4, 4) MI3_CASE(4, 7) MI3_CASE(4, 10)
+#undef MI3_CASE
+    return hipErrorInvalidValue;
+}
+
+template <int KW, int PF>
+struct SweepDisp {
+    static hipError_t run(const SweepArgs& a, hipStream_t st) { return sweep_kw<KW, PF>(a, st); }
+};
+template <int KW, int PF>
+struct EnergyDisp {
+    static hipError_t run(const EnergyArgs& a, hipStream_t st) { return energies_kw<KW, PF>(a, st); }
+};
+}  // namespace
+
+hipError_t launch_sweep(const SweepArgs& a, hipStream_t st) {
+    if (a.nwalk % SWEEP_WALKERS_PER_BLOCK || a.q > MAX_Q || a.L > MAX_L || a.L < 2)
+        return hipErrorInvalidValue;
+    return dispatch<SweepDisp>(a, st, a.nchunk, a.q);
+}
+
+hipError_t launch_energies(const EnergyArgs& a, hipStream_t st) {
+    if (a.q > MAX_Q || a.L > MAX_L || a.L < 2) return hipErrorInvalidValue;
+    if (a.nseq == 0) return hipSuccess;
+    return dispatch<EnergyDisp>(a, st, a.nchunk, a.q);
+}
+
+}  // namespace mi3

@@ -1,0 +1,780 @@
+// libmi3hip.so: context, named buffers, stream, tickets, profiling and the
+// C ABI declared in include/mi3hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "mi3hip.h"
+#include "launch.h"
+#include "k_misc.hip"
+#include "k_newton.hip"
+#include "k_bicount.hip"
+
+using namespace mi3;
+
+namespace {
+
+struct Buf {
+    void* d = nullptr;
+    size_t bytes = 0;
+    uint64_t version = 0;
+};
+
+struct Pinned {
+    void* h = nullptr;
+    size_t cap = 0;
+    bool busy = false;
+};
+
+struct Ticket {
+    int pin = -1;
+    size_t bytes = 0;
+    hipEvent_t ev = nullptr;
+    bool live = false;
+};
+
+struct ProfRec {
+    std::string name;
+    hipEvent_t a, b;
+    size_t bytes;
+};
+
+struct Chunked {
+    float* d = nullptr;
+    uint64_t version = ~0ull;
+};
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct mi3_ctx {
+    int dev = 0;
+    uint32_t L = 0, q = 0, nP = 0, N = 0, nchunk = 0;
+    uint64_t seed = 0;
+    uint32_t k0 = 0, k1 = 0;
+    hipStream_t stream = nullptr;
+    std::map<std::string, Buf> bufs;
+    std::string err;
+    // MCMC
+    bool mcmc = false;
+    uint32_t nsteps = 0;
+    uint64_t step = 0;
+    uint32_t walker_base = 0;
+    uint32_t indep_calls = 0;
+    uint32_t* pos = nullptr;
+    size_t pos_cap = 0;
+    // derived device data
+    std::map<std::string, Chunked> chunked;  // chunked layouts keyed by source buffer
+    double* red = nullptr;                    // reduction partials
+    uint32_t* wm_units = nullptr;
+    uint32_t n_units = 0;
+    uint8_t* seq1 = nullptr;                  // one sequence (fill_seqs)
+    // host staging
+    std::vector<Pinned> pins;
+    std::vector<Ticket> tickets;
+    // profiling
+    bool profile = false;
+    hipEvent_t t0 = nullptr;  // time origin of EVT records
+    std::vector<ProfRec> prof;
+};
+
+namespace {
+
+constexpr uint32_t RED_BLOCKS = 512;
+
+int fail(mi3_ctx* c, int code, const std::string& msg) {
+    if (c)
+        c->err = msg;
+    else
+        g_create_error = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(ctx, MI3_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+Buf* find(mi3_ctx* c, const char* name) {
+    if (!name) return nullptr;
+    auto it = c->bufs.find(name);
+    return it == c->bufs.end() ? nullptr : &it->second;
+}
+
+int need(mi3_ctx* c, const char* name, size_t bytes, Buf** out) {
+    Buf* b = find(c, name);
+    if (!b) return fail(c, MI3_ENOBUF, std::string("no buffer '") + (name ? name : "(null)") + "'");
+    if (b->bytes < bytes)
+        return fail(c, MI3_EINVAL,
+                    std::string("buffer '") + name + "' holds " + std::to_string(b->bytes) +
+                        " bytes, " + std::to_string(bytes) + " needed");
+    *out = b;
+    return MI3_OK;
+}
+
+// Row stride (bytes) of a site-major seq buffer.
+size_t seq_ld(mi3_ctx* c, Buf* b) { return c->L ? b->bytes / c->L : 0; }
+
+struct Scope {
+    mi3_ctx* c;
+    const char* name;
+    size_t bytes;
+    hipEvent_t a = nullptr;
+    Scope(mi3_ctx* c_, const char* n, size_t by) : c(c_), name(n), bytes(by) {
+        if (c->profile) {
+            hipEventCreate(&a);
+            hipEventRecord(a, c->stream);
+        }
+    }
+    ~Scope() {
+        if (c->profile) {
+            hipEvent_t b;
+            hipEventCreate(&b);
+            hipEventRecord(b, c->stream);
+            c->prof.push_back({name, a, b, bytes});
+        }
+    }
+};
+
+uint32_t grid1(uint64_t n, uint32_t bs = 256) { return (uint32_t)((n + bs - 1) / bs); }
+
+int fill_u32_dev(mi3_ctx* c, void* p, uint32_t val, uint64_t nwords) {
+    if (nwords == 0) return MI3_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid1(nwords)), dim3(256), 0, c->stream, (uint32_t*)p, val,
+                       nwords);
+    HIPCHK(c, hipGetLastError());
+    return MI3_OK;
+}
+
+// Chunked symmetric copy of a packed coupling buffer, rebuilt when stale.
+int chunked_of(mi3_ctx* c, const char* name, float** out) {
+    Buf* b;
+    size_t qq = (size_t)c->q * c->q;
+    int rc = need(c, name, (size_t)c->nP * qq * sizeof(float), &b);
+    if (rc) return rc;
+    Chunked& ch = c->chunked[name];
+    const uint32_t chf = chunk_floats(c->q);
+    uint64_t total = (uint64_t)c->L * c->nchunk * chf;
+    if (!ch.d) HIPCHK(c, hipMalloc(&ch.d, total * sizeof(float)));
+    if (ch.version != b->version) {
+        Scope s(c, "unpack_chunked", total * 4);
+        hipLaunchKernelGGL(k_unpack_chunked, dim3(grid1(total)), dim3(256), 0, c->stream,
+                           (const float*)b->d, ch.d, c->L, c->q, c->nchunk, chf, total);
+        HIPCHK(c, hipGetLastError());
+        ch.version = b->version;
+    }
+    *out = ch.d;
+    return MI3_OK;
+}
+
+int pin_get(mi3_ctx* c, size_t bytes) {
+    for (size_t i = 0; i < c->pins.size(); i++)
+        if (!c->pins[i].busy && c->pins[i].cap >= bytes) {
+            c->pins[i].busy = true;
+            return (int)i;
+        }
+    Pinned p;
+    p.cap = bytes < 4096 ? 4096 : bytes;
+    if (hipHostMalloc(&p.h, p.cap, hipHostMallocDefault) != hipSuccess) return -1;
+    p.busy = true;
+    c->pins.push_back(p);
+    return (int)c->pins.size() - 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mi3_version(void) { return "mi3hip 0.1 (gfx950)"; }
+
+int mi3_device_count(int* count) {
+    if (!count) return MI3_EINVAL;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail(nullptr, MI3_EHIP, hipGetErrorString(e));
+    }
+    return MI3_OK;
+}
+
+int mi3_device_info(int device, char* buf, size_t buflen) {
+    hipDeviceProp_t p;
+    hipError_t e = hipGetDeviceProperties(&p, device);
+    if (e != hipSuccess) return fail(nullptr, MI3_EHIP, hipGetErrorString(e));
+    snprintf(buf, buflen,
+             "  Device '%s' (%s):\n    Compute Units: %d\n    Clock (kHz): %d\n"
+             "    Global Mem Size: %zu\n    L2 Cache Size: %d\n    LDS per block: %zu\n",
+             p.name, p.gcnArchName, p.multiProcessorCount, p.clockRate, (size_t)p.totalGlobalMem,
+             p.l2CacheSize, (size_t)p.sharedMemPerBlock);
+    return MI3_OK;
+}
+
+int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t seed, int profile,
+               mi3_ctx** out) {
+    if (!out) return fail(nullptr, MI3_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (L < 2 || q < 2) return fail(nullptr, MI3_EINVAL, "need L >= 2 and q >= 2");
+    if (q > MAX_Q || L > MAX_L)
+        return fail(nullptr, MI3_EUNSUP,
+                    "this build supports q <= " + std::to_string(MAX_Q) + " and L <= " +
+                        std::to_string(MAX_L));
+    if (nwalkers % 512 != 0)
+        return fail(nullptr, MI3_EINVAL, "nwalkers/ngpus must be a multiple of 512");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(nullptr, MI3_EHIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(nullptr, MI3_EINVAL, "no such device");
+    if (hipSetDevice(device) != hipSuccess) return fail(nullptr, MI3_EHIP, "hipSetDevice failed");
+    mi3_ctx* c = new mi3_ctx;
+    c->dev = device;
+    c->L = L;
+    c->q = q;
+    c->nP = L * (L - 1) / 2;
+    c->N = nwalkers;
+    c->nchunk = (L + CH - 1) / CH;
+    c->seed = seed;
+    c->k0 = (uint32_t)seed;
+    c->k1 = (uint32_t)(seed >> 32);
+    c->profile = profile != 0;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(nullptr, MI3_EHIP, "stream creation failed");
+    }
+    if (c->profile) {
+        hipEventCreate(&c->t0);
+        hipEventRecord(c->t0, c->stream);
+    }
+    // weighted-marginal work units: (row i, chunk c) with chunk holding some m > i
+    std::vector<uint32_t> units;
+    for (uint32_t i = 0; i + 1 < L; i++)
+        for (uint32_t ch = (i + 1) / CH; ch < c->nchunk; ch++) units.push_back((i << 8) | ch);
+    c->n_units = (uint32_t)units.size();
+    if (hipMalloc(&c->wm_units, units.size() * 4) != hipSuccess ||
+        hipMemcpy(c->wm_units, units.data(), units.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&c->red, RED_BLOCKS * 2 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->seq1, L) != hipSuccess) {
+        mi3_destroy(c);
+        return fail(nullptr, MI3_ENOMEM, "device allocation failed");
+    }
+    *out = c;
+    return MI3_OK;
+}
+
+int mi3_destroy(mi3_ctx* c) {
+    if (!c) return MI3_OK;
+    hipSetDevice(c->dev);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& kv : c->bufs) hipFree(kv.second.d);
+    for (auto& kv : c->chunked) hipFree(kv.second.d);
+    for (auto& p : c->pins) hipHostFree(p.h);
+    for (auto& t : c->tickets)
+        if (t.ev) hipEventDestroy(t.ev);
+    for (auto& r : c->prof) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    hipFree(c->pos);
+    hipFree(c->red);
+    hipFree(c->wm_units);
+    hipFree(c->seq1);
+    if (c->t0) hipEventDestroy(c->t0);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return MI3_OK;
+}
+
+const char* mi3_last_error(mi3_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int mi3_buf_alloc(mi3_ctx* c, const char* name, size_t nbytes) {
+    if (!c || !name) return MI3_EINVAL;
+    if (find(c, name)) return fail(c, MI3_ESTATE, std::string("buffer '") + name + "' exists");
+    Buf b;
+    b.bytes = nbytes;
+    // pad so that 16-byte vector loads at the tail stay in bounds
+    size_t alloc = ((nbytes + 255) / 256) * 256 + 256;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipMalloc(&b.d, alloc));
+    int rc = fill_u32_dev(c, b.d, 0u, alloc / 4);
+    if (rc) return rc;
+    c->bufs[name] = b;
+    return MI3_OK;
+}
+
+int mi3_buf_exists(mi3_ctx* c, const char* name) { return (c && find(c, name)) ? 1 : 0; }
+
+int mi3_buf_size(mi3_ctx* c, const char* name, size_t* nbytes) {
+    Buf* b = c ? find(c, name) : nullptr;
+    if (!b) return c ? fail(c, MI3_ENOBUF, "no such buffer") : MI3_EINVAL;
+    *nbytes = b->bytes;
+    return MI3_OK;
+}
+
+int mi3_buf_write(mi3_ctx* c, const char* name, const void* src, size_t nbytes, size_t offset) {
+    if (!c || (!src && nbytes)) return MI3_EINVAL;
+    Buf* b;
+    int rc = need(c, name, offset + nbytes, &b);
+    if (rc) return rc;
+    if (nbytes == 0) return MI3_OK;
+    Scope s(c, "setBuf", nbytes);
+    HIPCHK(c, hipMemcpyAsync((char*)b->d + offset, src, nbytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    b->version++;
+    return MI3_OK;
+}
+
+int mi3_buf_copy(mi3_ctx* c, const char* dst, const char* src, size_t nbytes) {
+    if (!c) return MI3_EINVAL;
+    Buf *bd, *bs;
+    int rc = need(c, dst, nbytes, &bd);
+    if (rc) return rc;
+    rc = need(c, src, nbytes, &bs);
+    if (rc) return rc;
+    Scope s(c, "setBuf", nbytes);
+    HIPCHK(c, hipMemcpyAsync(bd->d, bs->d, nbytes, hipMemcpyDeviceToDevice, c->stream));
+    bd->version++;
+    return MI3_OK;
+}
+
+int mi3_buf_read_async(mi3_ctx* c, const char* name, size_t nbytes, size_t offset, int* ticket) {
+    if (!c || !ticket) return MI3_EINVAL;
+    Buf* b;
+    int rc = need(c, name, offset + nbytes, &b);
+    if (rc) return rc;
+    int pin = pin_get(c, nbytes);
+    if (pin < 0) return fail(c, MI3_ENOMEM, "pinned host allocation failed");
+    Ticket t;
+    t.pin = pin;
+    t.bytes = nbytes;
+    t.live = true;
+    HIPCHK(c, hipEventCreateWithFlags(&t.ev, hipEventDisableTiming));
+    {
+        Scope s(c, "getBuf", nbytes);
+        if (nbytes)
+            HIPCHK(c, hipMemcpyAsync(c->pins[pin].h, (char*)b->d + offset, nbytes,
+                                     hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipEventRecord(t.ev, c->stream));
+    for (size_t i = 0; i < c->tickets.size(); i++)
+        if (!c->tickets[i].live) {
+            c->tickets[i] = t;
+            *ticket = (int)i;
+            return MI3_OK;
+        }
+    c->tickets.push_back(t);
+    *ticket = (int)c->tickets.size() - 1;
+    return MI3_OK;
+}
+
+int mi3_ticket_wait(mi3_ctx* c, int ticket, void* dst, size_t nbytes) {
+    if (!c || ticket < 0 || (size_t)ticket >= c->tickets.size() || !c->tickets[ticket].live)
+        return c ? fail(c, MI3_EINVAL, "invalid ticket") : MI3_EINVAL;
+    Ticket& t = c->tickets[ticket];
+    HIPCHK(c, hipEventSynchronize(t.ev));
+    size_t n = nbytes < t.bytes ? nbytes : t.bytes;
+    if (dst && n) memcpy(dst, c->pins[t.pin].h, n);
+    c->pins[t.pin].busy = false;
+    hipEventDestroy(t.ev);
+    t.ev = nullptr;
+    t.live = false;
+    return MI3_OK;
+}
+
+int mi3_buf_fill_u32(mi3_ctx* c, const char* name, uint32_t pattern) {
+    if (!c) return MI3_EINVAL;
+    Buf* b;
+    int rc = need(c, name, 0, &b);
+    if (rc) return rc;
+    Scope s(c, "fill_buffer", b->bytes);
+    rc = fill_u32_dev(c, b->d, pattern, (b->bytes + 3) / 4);
+    if (rc) return rc;
+    b->version++;
+    return MI3_OK;
+}
+
+int mi3_synchronize(mi3_ctx* c) {
+    if (!c) return MI3_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MI3_OK;
+}
+
+int mi3_fill_seqs(mi3_ctx* c, const char* seqbuf, const uint8_t* seq) {
+    if (!c || !seq) return MI3_EINVAL;
+    Buf* b;
+    int rc = need(c, seqbuf, (size_t)c->L, &b);
+    if (rc) return rc;
+    size_t ld = seq_ld(c, b);
+    HIPCHK(c, hipMemcpyAsync(c->seq1, seq, c->L, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Scope s(c, "fillSeqs", b->bytes);
+    uint64_t n = (uint64_t)c->L * ld;
+    hipLaunchKernelGGL(k_fill_seqs, dim3(grid1(n)), dim3(256), 0, c->stream, (uint8_t*)b->d,
+                       (uint32_t)ld, (uint32_t)ld, (const uint8_t*)c->seq1, c->L);
+    HIPCHK(c, hipGetLastError());
+    b->version++;
+    return MI3_OK;
+}
+
+int mi3_init_mcmc(mi3_ctx* c, uint32_t nsteps, uint64_t rng_offset, uint64_t rng_span) {
+    if (!c) return MI3_EINVAL;
+    if (c->mcmc) return fail(c, MI3_ESTATE, "Already initialized MCMC");
+    if (nsteps == 0) return fail(c, MI3_EINVAL, "nsteps must be positive");
+    c->mcmc = true;
+    c->nsteps = nsteps;
+    c->walker_base = rng_span ? (uint32_t)(rng_offset / rng_span) * c->N : 0u;
+    if (c->pos_cap < nsteps) {
+        hipFree(c->pos);
+        c->pos = nullptr;
+        HIPCHK(c, hipMalloc(&c->pos, (size_t)nsteps * 4));
+        c->pos_cap = nsteps;
+    }
+    return MI3_OK;
+}
+
+int mi3_run_mcmc(mi3_ctx* c, float beta) {
+    if (!c) return MI3_EINVAL;
+    if (!c->mcmc) return fail(c, MI3_ESTATE, "MCMC not initialized");
+    Buf* sb;
+    int rc = need(c, "seq main", (size_t)c->L * c->N, &sb);
+    if (rc) return rc;
+    float* Ju;
+    rc = chunked_of(c, "J", &Ju);
+    if (rc) return rc;
+    {
+        Scope s(c, "positions", (size_t)c->nsteps * 4);
+        hipLaunchKernelGGL(k_positions, dim3(grid1(c->nsteps)), dim3(256), 0, c->stream, c->pos,
+                           c->nsteps, c->L, c->k0, c->k1, c->step);
+        HIPCHK(c, hipGetLastError());
+    }
+    SweepArgs a;
+    a.Ju = Ju;
+    a.seq = (uint8_t*)sb->d;
+    a.pos = c->pos;
+    a.ld = (uint32_t)seq_ld(c, sb);
+    a.nwalk = c->N;
+    a.L = c->L;
+    a.q = c->q;
+    a.nchunk = c->nchunk;
+    a.nsteps = c->nsteps;
+    a.k0 = c->k0;
+    a.k1 = c->k1;
+    a.walker_base = c->walker_base;
+    a.step0 = c->step;
+    a.nbeta = -beta;
+    {
+        Scope s(c, "mcmc", (size_t)c->N * c->nsteps * (c->L + 1));
+        HIPCHK(c, launch_sweep(a, c->stream));
+    }
+    c->step += c->nsteps + (c->nsteps & 1u);  // keep the global step even
+    sb->version++;
+    return MI3_OK;
+}
+
+int mi3_mcmc_step_count(mi3_ctx* c, uint64_t* steps) {
+    if (!c || !steps) return MI3_EINVAL;
+    *steps = c->step;
+    return MI3_OK;
+}
+
+int mi3_gen_indep(mi3_ctx* c, const char* seqbuf) {
+    if (!c) return MI3_EINVAL;
+    Buf *sb, *cp;
+    int rc = need(c, seqbuf, (size_t)c->L, &sb);
+    if (rc) return rc;
+    rc = need(c, "cprob", (size_t)c->L * (c->q - 1) * 4, &cp);
+    if (rc) return rc;
+    uint32_t ld = (uint32_t)seq_ld(c, sb);
+    Scope s(c, "gen_indep", sb->bytes);
+    hipLaunchKernelGGL(k_gen_indep, dim3(grid1(ld)), dim3(256), 0, c->stream, (uint8_t*)sb->d, ld,
+                       ld, (const float*)cp->d, c->L, c->q, c->k0, c->k1, c->indep_calls,
+                       c->walker_base);
+    HIPCHK(c, hipGetLastError());
+    c->indep_calls++;
+    sb->version++;
+    return MI3_OK;
+}
+
+int mi3_calc_energies(mi3_ctx* c, const char* seqbuf, const char* Jbuf, const char* Ebuf,
+                      uint32_t nseq) {
+    if (!c) return MI3_EINVAL;
+    Buf *sb, *eb;
+    int rc = need(c, seqbuf, (size_t)c->L, &sb);
+    if (rc) return rc;
+    size_t ld = seq_ld(c, sb);
+    if (nseq > ld) return fail(c, MI3_EINVAL, "nseq exceeds the sequence buffer");
+    rc = need(c, Ebuf, (size_t)nseq * 4, &eb);
+    if (rc) return rc;
+    float* Ju;
+    rc = chunked_of(c, Jbuf, &Ju);
+    if (rc) return rc;
+    EnergyArgs a;
+    a.Ju = Ju;
+    a.seq = (const uint8_t*)sb->d;
+    a.E = (float*)eb->d;
+    a.ld = (uint32_t)ld;
+    a.nseq = nseq;
+    a.L = c->L;
+    a.q = c->q;
+    a.nchunk = c->nchunk;
+    Scope s(c, "getEnergies", (size_t)nseq * (c->L + 4));
+    HIPCHK(c, launch_energies(a, c->stream));
+    eb->version++;
+    return MI3_OK;
+}
+
+int mi3_calc_bicounts(mi3_ctx* c, const char* seqbuf, uint32_t nseq) {
+    if (!c) return MI3_EINVAL;
+    Buf *sb, *cb;
+    int rc = need(c, seqbuf, (size_t)c->L, &sb);
+    if (rc) return rc;
+    size_t ld = seq_ld(c, sb);
+    if (nseq > ld) return fail(c, MI3_EINVAL, "nseq exceeds the sequence buffer");
+    if (ld % 16) return fail(c, MI3_EINVAL, "sequence buffer row stride must be a multiple of 16");
+    rc = need(c, "bicount", (size_t)c->nP * c->q * c->q * 4, &cb);
+    if (rc) return rc;
+    uint32_t Lq = c->L * c->q;
+    uint32_t nblk = (Lq + 63) / 64;
+    uint32_t nwav = nblk * (nblk + 1) / 2;
+    Scope s(c, "calcBicounts", (size_t)nseq * c->L);
+    hipLaunchKernelGGL(k_bicounts_mfma, dim3(nwav), dim3(64), 0, c->stream, (const uint8_t*)sb->d,
+                       (uint32_t)ld, nseq, c->L, c->q, nblk, (uint32_t*)cb->d);
+    HIPCHK(c, hipGetLastError());
+    cb->version++;
+    return MI3_OK;
+}
+
+int mi3_bicounts_to_bimarg(mi3_ctx* c, uint32_t nseq) {
+    if (!c) return MI3_EINVAL;
+    Buf *cb, *bb;
+    size_t n = (size_t)c->nP * c->q * c->q;
+    int rc = need(c, "bicount", n * 4, &cb);
+    if (rc) return rc;
+    rc = need(c, "bi", n * 4, &bb);
+    if (rc) return rc;
+    Scope s(c, "bicounts_to_bimarg", n * 8);
+    hipLaunchKernelGGL(k_bicounts_to_bimarg, dim3(grid1(n)), dim3(256), 0, c->stream,
+                       (const uint32_t*)cb->d, (float*)bb->d, nseq, (uint64_t)n);
+    HIPCHK(c, hipGetLastError());
+    bb->version++;
+    return MI3_OK;
+}
+
+int mi3_weighted_marg(mi3_ctx* c, const char* seqbuf, const char* wbuf, uint32_t nseq) {
+    if (!c) return MI3_EINVAL;
+    Buf *sb, *wb, *bb;
+    int rc = need(c, seqbuf, (size_t)c->L, &sb);
+    if (rc) return rc;
+    size_t ld = seq_ld(c, sb);
+    if (nseq > ld) return fail(c, MI3_EINVAL, "nseq exceeds the sequence buffer");
+    if (ld % 16) return fail(c, MI3_EINVAL, "sequence buffer row stride must be a multiple of 16");
+    rc = need(c, wbuf, (size_t)nseq * 4, &wb);
+    if (rc) return rc;
+    size_t qq = (size_t)c->q * c->q;
+    rc = need(c, "bi", c->nP * qq * 4, &bb);
+    if (rc) return rc;
+    size_t lds = qq * CH * 4;
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_weighted_marg),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    Scope s(c, "weightedMarg", (size_t)nseq * (c->L + 4));
+    hipLaunchKernelGGL(k_weighted_marg, dim3(c->n_units), dim3(256), lds, c->stream,
+                       (const uint8_t*)sb->d, (uint32_t)ld, nseq, (const float*)wb->d, c->L, c->q,
+                       c->nchunk, (const uint32_t*)c->wm_units, (float*)bb->d);
+    HIPCHK(c, hipGetLastError());
+    bb->version++;
+    return MI3_OK;
+}
+
+int mi3_renormalize_bimarg(mi3_ctx* c) {
+    if (!c) return MI3_EINVAL;
+    Buf* bb;
+    size_t qq = (size_t)c->q * c->q;
+    int rc = need(c, "bi", c->nP * qq * 4, &bb);
+    if (rc) return rc;
+    Scope s(c, "renormalize_bimarg", c->nP * qq * 8);
+    hipLaunchKernelGGL(k_renormalize, dim3((c->nP + 3) / 4), dim3(256), 0, c->stream,
+                       (float*)bb->d, c->nP, (uint32_t)qq);
+    HIPCHK(c, hipGetLastError());
+    bb->version++;
+    return MI3_OK;
+}
+
+static int reduce(mi3_ctx* c, const char* buf, uint32_t n, int op, const char* outname) {
+    Buf *b, *o;
+    int rc = need(c, buf, (size_t)n * 4, &b);
+    if (rc) return rc;
+    rc = need(c, outname, op == 0 ? 4 : 8, &o);
+    if (rc) return rc;
+    uint32_t nb = grid1(n);
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+    if (nb == 0) nb = 1;
+    Scope s(c, op == 0 ? "min_buf" : "weight_statistics", (size_t)n * 4);
+    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(256), 0, c->stream, (const float*)b->d, n,
+                       op, c->red);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->stream, (const double*)c->red,
+                       nb, op, (float*)o->d);
+    HIPCHK(c, hipGetLastError());
+    o->version++;
+    return MI3_OK;
+}
+
+int mi3_min_buf(mi3_ctx* c, const char* buf, uint32_t n) {
+    if (!c) return MI3_EINVAL;
+    return reduce(c, buf, n, 0, "minout");
+}
+
+int mi3_weight_stats(mi3_ctx* c, const char* wbuf, uint32_t n) {
+    if (!c) return MI3_EINVAL;
+    return reduce(c, wbuf, n, 1, "weightstats");
+}
+
+int mi3_dE_to_weights(mi3_ctx* c, const char* Ebuf, const char* wbuf, uint32_t n, float ref) {
+    if (!c) return MI3_EINVAL;
+    Buf *e, *w;
+    int rc = need(c, Ebuf, (size_t)n * 4, &e);
+    if (rc) return rc;
+    rc = need(c, wbuf, (size_t)n * 4, &w);
+    if (rc) return rc;
+    Scope s(c, "dE_to_weights", (size_t)n * 8);
+    hipLaunchKernelGGL(k_dE_to_weights, dim3(grid1(n)), dim3(256), 0, c->stream,
+                       (const float*)e->d, (float*)w->d, ref, n);
+    HIPCHK(c, hipGetLastError());
+    w->version++;
+    return MI3_OK;
+}
+
+int mi3_fixed_beta_weights(mi3_ctx* c, const char* Ebuf, const char* wbuf, uint32_t n, float ref,
+                           float beta) {
+    if (!c) return MI3_EINVAL;
+    Buf *e, *w;
+    int rc = need(c, Ebuf, (size_t)n * 4, &e);
+    if (rc) return rc;
+    rc = need(c, wbuf, (size_t)n * 4, &w);
+    if (rc) return rc;
+    Scope s(c, "fixed_beta_weights", (size_t)n * 8);
+    hipLaunchKernelGGL(k_fixed_beta_weights, dim3(grid1(n)), dim3(256), 0, c->stream,
+                       (const float*)e->d, (float*)w->d, ref, beta - 1.f, n);
+    HIPCHK(c, hipGetLastError());
+    w->version++;
+    return MI3_OK;
+}
+
+int mi3_add_float_bufs(mi3_ctx* c, const char* dst, const char* src, uint32_t n) {
+    if (!c) return MI3_EINVAL;
+    Buf *d, *s_;
+    int rc = need(c, dst, (size_t)n * 4, &d);
+    if (rc) return rc;
+    rc = need(c, src, (size_t)n * 4, &s_);
+    if (rc) return rc;
+    Scope s(c, "addbuf", (size_t)n * 12);
+    hipLaunchKernelGGL(k_add_floats, dim3(grid1(n)), dim3(256), 0, c->stream, (float*)d->d,
+                       (const float*)s_->d, (uint64_t)n);
+    HIPCHK(c, hipGetLastError());
+    d->version++;
+    return MI3_OK;
+}
+
+static int update_reg(mi3_ctx* c, const char* Jbuf, float gamma, float pc, float lam, int mode,
+                      const char* label) {
+    Buf *bt, *bi, *J, *dJ;
+    size_t n = (size_t)c->nP * c->q * c->q * 4;
+    int rc = need(c, "bi target", n, &bt);
+    if (rc) return rc;
+    rc = need(c, "bi", n, &bi);
+    if (rc) return rc;
+    rc = need(c, "J", n, &J);
+    if (rc) return rc;
+    rc = need(c, Jbuf, n, &dJ);
+    if (rc) return rc;
+    Scope s(c, label, n * (mode >= 3 ? 5 : 4));
+    hipLaunchKernelGGL(k_update_reg, dim3((c->nP + 3) / 4), dim3(256), 0, c->stream,
+                       (const float*)bt->d, (const float*)bi->d, (const float*)J->d, (float*)dJ->d,
+                       c->nP, c->q, gamma, pc, lam, mode);
+    HIPCHK(c, hipGetLastError());
+    dJ->version++;
+    return MI3_OK;
+}
+
+int mi3_update_J(mi3_ctx* c, float gamma, float pc, const char* Jbuf) {
+    if (!c) return MI3_EINVAL;
+    return update_reg(c, Jbuf ? Jbuf : "dJ", gamma, pc, 0.f, 0, "updateJ");
+}
+
+int mi3_reg_l1z(mi3_ctx* c, float gamma, float pc, float lJ) {
+    if (!c) return MI3_EINVAL;
+    return update_reg(c, "dJ", gamma, pc, lJ, 1, "reg_l1z");
+}
+
+int mi3_reg_l2z(mi3_ctx* c, float gamma, float pc, float lJ) {
+    if (!c) return MI3_EINVAL;
+    return update_reg(c, "dJ", gamma, pc, lJ, 2, "reg_l2z");
+}
+
+int mi3_update_J_reg(mi3_ctx* c, float gamma, float pc, int reg, float lJ) {
+    if (!c) return MI3_EINVAL;
+    if (reg < 0 || reg > 2) return fail(c, MI3_EINVAL, "reg must be 0, 1 (l1z) or 2 (l2z)");
+    return update_reg(c, "dJ", gamma, pc, lJ, reg == 0 ? 0 : reg + 2, "updateJ_reg");
+}
+
+int mi3_profile_flush(mi3_ctx* c, const char* path) {
+    if (!c) return MI3_EINVAL;
+    FILE* f = path ? fopen(path, "a") : nullptr;
+    int n = 0;
+    size_t keep = 0;
+    for (size_t i = 0; i < c->prof.size(); i++) {
+        ProfRec& r = c->prof[i];
+        if (hipEventQuery(r.b) != hipSuccess) {
+            c->prof[keep++] = r;
+            continue;
+        }
+        float ms_a = 0.f, ms_b = 0.f;
+        hipEventElapsedTime(&ms_a, c->t0, r.a);
+        hipEventElapsedTime(&ms_b, c->t0, r.b);
+        if (f)
+            fprintf(f, "EVT %s %lld %lld %zu\n", r.name.c_str(), (long long)(ms_a * 1e6),
+                    (long long)(ms_b * 1e6), r.bytes);
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+        n++;
+    }
+    c->prof.resize(keep);
+    if (f) fclose(f);
+    return n;
+}
+
+int mi3_profile_kernel_ms(mi3_ctx* c, const char* name, double* avg_ms, int* count) {
+    if (!c || !name || !avg_ms || !count) return MI3_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0;
+    int n = 0;
+    for (auto& r : c->prof)
+        if (r.name == name) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+                tot += ms;
+                n++;
+            }
+        }
+    *avg_ms = n ? tot / n : 0.0;
+    *count = n;
+    return MI3_OK;
+}
+
+int mi3_profile_reset(mi3_ctx* c) {
+    if (!c) return MI3_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto& r : c->prof) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    c->prof.clear();
+    return MI3_OK;
+}
+
+}  // extern "C"
