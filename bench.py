@@ -74,6 +74,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--walkers', type=int, default=2 ** 18)
     ap.add_argument('--sweeps-per-step', type=int, default=8)
+    ap.add_argument('--wgsize', type=int, default=512,
+                    help="walkers per work-group of the sweep (128, 256, 512)")
     ap.add_argument('--no-cpu', action='store_true', help="skip the CPU baseline")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -87,7 +89,7 @@ def main():
     J = production_model(L, q, seed=1)
     s0 = np.random.default_rng(2).integers(0, q, size=(N, L)).astype('<u1')
     with tempfile.TemporaryDirectory() as d:
-        g = MCMCGPU((0, 0, None, None), L, q, N, 512, Path(d), 1024, 2, profile=True)
+        g = MCMCGPU((0, 0, None, None), L, q, N, a.wgsize, Path(d), 1024, 2, profile=True)
         g.initMCMC(nsteps, 0, 2 ** 63)
         g.setBuf('J', J)
         g.setBuf('seq main', s0)
@@ -126,6 +128,7 @@ def main():
                                "Potts model h~N(0,1) J~N(0,0.05) zero-sum gauge folded "
                                "fieldless float32",
                    "L": L, "q": q, "walkers": N, "sweeps_per_step": a.sweeps_per_step,
+                   "walkers_per_workgroup": a.wgsize,
                    "mean_energy_after": float(np.mean(E))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

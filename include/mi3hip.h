@@ -112,6 +112,13 @@ int mi3_fill_seqs(mi3_ctx *ctx, const char *seqbuf, const uint8_t *seq);
 
 /* ---- MCMC ---------------------------------------------------------------- */
 
+/* Launch shape of the sweep: walkers per work-group (128, 256 or 512; other
+ * powers of two are rounded into that range).  Takes the role of the
+ * reference's work-group size option (WGSIZE compile define,
+ * mcmcGPU.py:1080-1093; wgsize_heuristic mcmcGPU.py:1121-1132; --wgsize
+ * Mi3.py:115).  Sequences longer than 256 always use 128. */
+int mi3_set_wgsize(mi3_ctx *ctx, uint32_t wgsize);
+
 /* Set nsteps per runMCMC call and the walker-stream offset (initMCMC,
  * mcmcGPU.py:271-283, and _initMCMC_RNG 410-445).  `rng_offset` /
  * `rng_span` select a disjoint walker-index range per device, as the

@@ -24,19 +24,6 @@ namespace mi3 {
 // arrays of it out of registers).
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-// Chunks are padded to PF * NT float4, so staging is unconditional.
-template <int PF, int NT>
-__device__ __forceinline__ void chunk_load(f4v (&pf)[PF], const f4v* __restrict__ src, int tid) {
-#pragma unroll
-    for (int r = 0; r < PF; r++) pf[r] = src[tid + r * NT];
-}
-
-template <int PF, int NT>
-__device__ __forceinline__ void chunk_store(const f4v (&pf)[PF], f4v* dst, int tid) {
-#pragma unroll
-    for (int r = 0; r < PF; r++) dst[tid + r * NT] = pf[r];
-}
-
 // w[idx] for a wave-uniform idx, as a select chain over registers.  The
 // empty asm keeps the compiler from turning the chain back into a dynamic
 // index (which would move the state array to scratch memory).
@@ -52,6 +39,143 @@ __device__ __forceinline__ uint32_t select_word(const uint32_t (&w)[KW], uint32_
     return r;
 }
 
+// Word k of a walker's packed state.  Static indices address registers
+// directly; for KW >= 3 the (unrolled) index goes through the opaque select
+// chain so the state array stays in registers.
+template <int KW>
+__device__ __forceinline__ uint32_t state_word(const uint32_t (&w)[KW], uint32_t k) {
+    if constexpr (KW <= 2)
+        return (KW == 1 || k == 0) ? w[0] : w[KW - 1];
+    else
+        return select_word<KW>(w, k);
+}
+
+__device__ __forceinline__ float swz_xor_rt(float v, int m) {
+    switch (m) {
+        case 16: return swz_xor<16>(v);
+        case 8: return swz_xor<8>(v);
+        case 4: return swz_xor<4>(v);
+        case 2: return swz_xor<2>(v);
+        default: return swz_xor<1>(v);
+    }
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+// Wave-wide lane masks: bit `lane` set when (lane & m) != 0.
+__device__ __forceinline__ constexpr uint64_t lane_bit_mask(int m) {
+    return m == 16 ? 0xFFFF0000FFFF0000ull
+         : m == 8  ? 0xFF00FF00FF00FF00ull
+         : m == 4  ? 0xF0F0F0F0F0F0F0F0ull
+         : m == 2  ? 0xCCCCCCCCCCCCCCCCull
+                   : 0xAAAAAAAAAAAAAAAAull;
+}
+
+// mask[lane] ? a : b as one v_cndmask.  Written as asm so that the
+// optimizer cannot turn a select between two elements of one register array
+// back into a lane-varying array index (which it lowers to a long
+// compare/select chain).
+__device__ __forceinline__ float lane_select(uint64_t mask, float a, float b) {
+    float r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(mask));
+    return r;
+}
+
+// Reduce S per-lane partial sums over the 32 lanes of each half-wave at once
+// (butterfly with operand exchange): afterwards lane j holds the full sum of
+// slot j >> (5 - log2 S).  Costs about 4*S + 5 operations instead of the
+// 10*S of S independent butterflies.
+template <int S>
+__device__ __forceinline__ float transposed_sum(float (&v)[S]) {
+    int n = S;
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+        const int m = 16 >> l;
+        if (n > 1) {
+            const int h = n / 2;
+            const uint64_t mk = lane_bit_mask(m);
+#pragma unroll
+            for (int s = 0; s < S / 2; s++) {
+                if (s < h) {
+                    float send = lane_select(mk, v[s], v[s + h]);
+                    float keep = lane_select(mk, v[s + h], v[s]);
+                    v[s] = keep + swz_xor_rt(send, m);
+                }
+            }
+            n = h;
+        } else {
+            v[0] += swz_xor_rt(v[0], m);
+        }
+    }
+    return v[0];
+}
+
+// Buffer resource for raw buffer loads (gfx9 dword3 = 0x00020000): 32-bit
+// offsets, so staging addresses need no 64-bit arithmetic.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+// Copy one padded chunk (PF * NT float4) from global memory straight into
+// LDS with buffer_load ... lds: lane l of wave w moves float4 number
+// k*NT + 64*w + l; the hardware writes the 64 float4 of a wave contiguously
+// at the wave-uniform LDS address.  No registers hold the data in flight;
+// completion is awaited with s_waitcnt vmcnt(0) before the next barrier.
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int PF, int NT>
+__device__ __forceinline__ void chunk_dma(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                          char* lds_buf, int wave) {
+#pragma unroll
+    for (int k = 0; k < PF; k++)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            r, (lds_void*)(lds_buf + (k * NT + wave * 64) * 16), 16, voff, soff + k * NT * 16, 0,
+            0);
+}
+
+// s_waitcnt vmcnt(0) (expcnt, lgkmcnt left at their maxima): every
+// vector-memory operation of this wave, including LDS DMA, has completed.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Value of `v` in lane (src + 32*half) of each half-wave, src a compile-time
+// lane number < 32 (ds_swizzle bitmask mode: and 0, or src).
+#define MI3_SWZ_CASE(n) \
+    case n:             \
+        return __builtin_amdgcn_ds_swizzle(v, (n) << 5);
+__device__ __forceinline__ int half_bcast(int v, int src) {
+    switch (src) {
+        MI3_SWZ_CASE(0) MI3_SWZ_CASE(1) MI3_SWZ_CASE(2) MI3_SWZ_CASE(3) MI3_SWZ_CASE(4)
+        MI3_SWZ_CASE(5) MI3_SWZ_CASE(6) MI3_SWZ_CASE(7) MI3_SWZ_CASE(8) MI3_SWZ_CASE(9)
+        MI3_SWZ_CASE(10) MI3_SWZ_CASE(11) MI3_SWZ_CASE(12) MI3_SWZ_CASE(13) MI3_SWZ_CASE(14)
+        MI3_SWZ_CASE(15) MI3_SWZ_CASE(16) MI3_SWZ_CASE(17) MI3_SWZ_CASE(18) MI3_SWZ_CASE(19)
+        MI3_SWZ_CASE(20) MI3_SWZ_CASE(21) MI3_SWZ_CASE(22) MI3_SWZ_CASE(23) MI3_SWZ_CASE(24)
+        MI3_SWZ_CASE(25) MI3_SWZ_CASE(26) MI3_SWZ_CASE(27) MI3_SWZ_CASE(28) MI3_SWZ_CASE(29)
+        MI3_SWZ_CASE(30) default: return __builtin_amdgcn_ds_swizzle(v, 31 << 5);
+    }
+}
+#undef MI3_SWZ_CASE
+
+template <int KW>
+__device__ __forceinline__ uint32_t select_word_opaque(const uint32_t (&w)[KW], uint32_t idx) {
+    if constexpr (KW == 1) return w[0];
+    if constexpr (KW == 2) return idx ? w[1] : w[0];
+    uint32_t r = w[0];
+#pragma unroll
+    for (int k = 1; k < KW; k++) {
+        uint32_t t = w[k];
+        asm("" : "+v"(t));
+        r = (idx == (uint32_t)k) ? t : r;
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t lane_select_u(uint64_t mask, uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(mask));
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // Metropolis sweep: nsteps single-site updates per walker.
 //
@@ -62,6 +186,13 @@ __device__ __forceinline__ uint32_t select_word(const uint32_t (&w)[KW], uint32_
 // even or odd g.  dE = sum_{m != pos} J_{pos,m}(b, s_m) - J_{pos,m}(a, s_m).
 // Restates metropolis/DeltaEnergy (mcmc.cl:360-458) with counter-based
 // streams in place of MWC64X.
+//
+// Per step: lane j draws the random numbers of walker slot j >> SH; each
+// lane then fetches, for every slot, the walker's current residue a (from
+// the lane owning site pos, ds_bpermute) and proposal b (ds_swizzle
+// broadcast), gathers J(a, s_m) and J(b, s_m) for its sites chunk by chunk,
+// and one transposed butterfly leaves each lane with the dE of its own slot;
+// it decides, and a ballot tells the owner lane of pos which walkers accepted.
 template <int KW, int S, int NWAVES, int PF>
 __global__ __launch_bounds__(64 * NWAVES) void k_sweep(
     const float* __restrict__ Ju, uint8_t* __restrict__ seq, const uint32_t* __restrict__ pos_list,
@@ -69,16 +200,22 @@ __global__ __launch_bounds__(64 * NWAVES) void k_sweep(
     uint32_t k1, uint64_t step0, uint32_t walker_base, float nbeta) {
     constexpr int NT = 64 * NWAVES;
     constexpr int K = 4 * KW;
+    constexpr int SH = 5 - ilog2(S);  // lane j reduces and decides slot j >> SH
+    static_assert((1 << ilog2(S)) == S && S <= 32, "S must be a power of two <= 32");
     extern __shared__ f4v lds4[];
-    float* lds = reinterpret_cast<float*>(lds4);
+    char* ldsb = reinterpret_cast<char*>(lds4);
 
     constexpr uint32_t chf = PF * NT * 4;  // padded chunk (floats)
+    constexpr uint32_t chb = chf * 4;      // padded chunk (bytes)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const uint32_t j = lane & 31;
     const bool upper = lane >= 32;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t wfirst = blockIdx.x * (2 * S * NWAVES) + (wave * 2 + (upper ? 1 : 0)) * S;
+    const uint32_t myslot = j >> SH;
+    const __amdgpu_buffer_rsrc_t rJ = make_rsrc(Ju, L * nchunk * chb);
+    const uint32_t voff = tid * 16;
 
     // walker state: residues of sites j + 32k
     uint32_t st[S][KW];
@@ -98,51 +235,40 @@ __global__ __launch_bounds__(64 * NWAVES) void k_sweep(
         asm volatile("" ::: "memory");
     }
 
-    uint32_t P[S];      // packed (u24 << 8 | b) for 32 steps, lane (t>>1) + 16*(t&1)
-    uint32_t cur[S];    // packed draw of the current step
-    uint32_t base[S];   // (a*q*32 + j) | (b*q*32 + j) << 16, LDS float index
+    uint32_t baseA[S], baseB[S];  // LDS byte offsets of rows a and b (+ lane column)
     float acc[S];
+    uint32_t Pe = 0, Po = 0;      // this lane's walker draws for steps 2h, 2h+1
 
-    f4v pf[PF];
-    // prologue: chunk 0 of step 0 into buffer 0
     if (nsteps == 0) return;
-    chunk_load<PF, NT>(pf, reinterpret_cast<const f4v*>(Ju + (size_t)pos_list[0] * nchunk * chf),
-                       tid);
-    chunk_store<PF, NT>(pf, lds4, tid);
+    chunk_dma<PF, NT>(rJ, voff, pos_list[0] * nchunk * chb, ldsb, wave);
+    wait_vmem();
     __syncthreads();
     uint32_t buf = 0;
-    const uint32_t qch = q * CH;
+    const uint32_t qch4 = q * CH * 4;
+    const uint32_t j4 = j * 4;
 
     for (uint32_t t = 0; t < nsteps; t++) {
         const uint32_t pos = pos_list[t];
         const uint32_t kp = pos >> 5, owner = pos & 31;
         const uint32_t kwp = kp >> 2, shp = (kp & 3) * 8;
-        const uint32_t tl = t & 31;
-        if (tl == 0) {
-            // refill the per-walker draws for steps t .. t+31 (global step even)
-            const uint64_t h = ((step0 + t) >> 1) + (j & 15);
-#pragma unroll
-            for (int s = 0; s < S; s++) {
-                uint4 r = philox4x32_10(make_uint4((uint32_t)h, (uint32_t)(h >> 32),
-                                                   walker_base + wfirst + s, TAG_WALK),
-                                        k0, k1);
-                uint32_t x = (j < 16) ? r.x : r.z;
-                uint32_t y = (j < 16) ? r.y : r.w;
-                P[s] = ((y >> 8) << 8) | mulhi_range(x, q);
-            }
+        if ((t & 1) == 0) {
+            const uint64_t h = (step0 + t) >> 1;
+            uint4 r = philox4x32_10(make_uint4((uint32_t)h, (uint32_t)(h >> 32),
+                                               walker_base + wfirst + myslot, TAG_WALK),
+                                    k0, k1);
+            Pe = ((r.y >> 8) << 8) | mulhi_range(r.x, q);
+            Po = ((r.w >> 8) << 8) | mulhi_range(r.z, q);
         }
-        const uint32_t rl = (tl >> 1) + ((tl & 1) << 4);
+        const uint32_t Pcur = (t & 1) ? Po : Pe;
+        const int bmine = (int)(Pcur & 0xffu);
+        const int own_addr = (int)((owner + (lane & 32)) * 4);
 #pragma unroll
         for (int s = 0; s < S; s++) {
-            uint32_t av = (select_word<KW>(st[s], kwp) >> shp) & 0xffu;
-            uint32_t aA = __builtin_amdgcn_readlane(av, owner);
-            uint32_t aB = __builtin_amdgcn_readlane(av, owner + 32);
-            uint32_t a = upper ? aB : aA;
-            uint32_t pA = __builtin_amdgcn_readlane(P[s], rl);
-            uint32_t pB = __builtin_amdgcn_readlane(P[s], rl + 32);
-            cur[s] = upper ? pB : pA;
-            uint32_t b = cur[s] & 0xffu;
-            base[s] = (a * qch + j) | ((b * qch + j) << 16);
+            uint32_t av = (select_word_opaque<KW>(st[s], kwp) >> shp) & 0xffu;
+            uint32_t a = (uint32_t)__builtin_amdgcn_ds_bpermute(own_addr, (int)av);
+            uint32_t b = (uint32_t)half_bcast(bmine, s << SH);
+            baseA[s] = __umul24(a, qch4) + j4;
+            baseB[s] = __umul24(b, qch4) + j4;
             acc[s] = 0.f;
         }
         // stream the couplings of site pos, chunk by chunk
@@ -154,36 +280,42 @@ __global__ __launch_bounds__(64 * NWAVES) void k_sweep(
                 if (has_next) {
                     uint32_t npos = last_chunk ? pos_list[t + 1] : pos;
                     uint32_t nc = last_chunk ? 0u : (uint32_t)k + 1;
-                    chunk_load<PF, NT>(
-                        pf, reinterpret_cast<const f4v*>(Ju + ((size_t)npos * nchunk + nc) * chf),
-                        tid);
+                    chunk_dma<PF, NT>(rJ, voff, (npos * nchunk + nc) * chb, ldsb + (buf ^ 1) * chb,
+                                      wave);
                 }
-                const float* cb = lds + buf * chf;
+                const char* cb = ldsb + buf * chb;
 #pragma unroll
                 for (int s = 0; s < S; s++) {
-                    uint32_t sm = (select_word<KW>(st[s], (uint32_t)k >> 2) >> ((k & 3) * 8)) & 0xffu;
-                    uint32_t off = sm * CH;
-                    float vA = cb[(base[s] & 0xffffu) + off];
-                    float vB = cb[(base[s] >> 16) + off];
+                    uint32_t off = ((state_word<KW>(st[s], (uint32_t)k >> 2) >> ((k & 3) * 8)) &
+                                    0xffu) * (CH * 4);
+                    float vA = *reinterpret_cast<const float*>(cb + baseA[s] + off);
+                    float vB = *reinterpret_cast<const float*>(cb + baseB[s] + off);
                     acc[s] += vB - vA;
                 }
-                if (has_next) chunk_store<PF, NT>(pf, lds4 + (buf ^ 1) * (chf / 4), tid);
+                wait_vmem();
                 __syncthreads();
                 buf ^= 1;
             }
         }
-        // accept / reject
+        // accept / reject: this lane decides for walker slot `myslot`
+        const float dE = transposed_sum<S>(acc);
+        const float u = (float)(Pcur >> 8) * 0x1.0p-24f;
+        const uint64_t amask = __ballot(__expf(nbeta * dE) > u);
+        const uint64_t own_m = (1ull << owner) | (1ull << (owner + 32));
 #pragma unroll
         for (int s = 0; s < S; s++) {
-            float dE = half_wave_sum(acc[s]);
-            float u = (float)(cur[s] >> 8) * 0x1.0p-24f;
-            const bool upd = (__expf(nbeta * dE) > u) && (j == owner);
-            const uint32_t b = cur[s] & 0xffu;
+            const uint64_t mA = ((amask >> (s << SH)) & 1ull) ? 0x00000000FFFFFFFFull : 0ull;
+            const uint64_t mB = ((amask >> ((s << SH) + 32)) & 1ull) ? 0xFFFFFFFF00000000ull : 0ull;
+            const uint64_t m = (mA | mB) & own_m;
+            if (m) {
+                const uint32_t bsh = (uint32_t)half_bcast(bmine, s << SH) << shp;
 #pragma unroll
-            for (int kw = 0; kw < KW; kw++) {
-                uint32_t nw = (st[s][kw] & ~(0xffu << shp)) | (b << shp);
-                asm volatile("" : "+v"(nw));
-                st[s][kw] = (upd && kwp == (uint32_t)kw) ? nw : st[s][kw];
+                for (int kw = 0; kw < KW; kw++) {
+                    if (kwp == (uint32_t)kw) {
+                        uint32_t nw = (st[s][kw] & ~(0xffu << shp)) | bsh;
+                        st[s][kw] = lane_select_u(m, nw, st[s][kw]);
+                    }
+                }
             }
         }
     }
@@ -220,7 +352,7 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
     const int lane = tid & 63;
     const uint32_t j = lane & 31;
     const bool upper = lane >= 32;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t wfirst = blockIdx.x * (2 * S * NWAVES) + (wave * 2 + (upper ? 1 : 0)) * S;
 
     uint32_t st[S][KW];
@@ -245,11 +377,13 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
     for (int s = 0; s < S; s++) acc[s] = 0.f;
     uint32_t base[S];
 
-    f4v pf[PF];
+    const __amdgpu_buffer_rsrc_t rJ = make_rsrc(Ju, L * nchunk * chf * 4);
+    const uint32_t voff = tid * 16;
+    char* ldsb = reinterpret_cast<char*>(lds4);
     if (L >= 2) {
         // first unit: row 0, chunk 0
-        chunk_load<PF, NT>(pf, reinterpret_cast<const f4v*>(Ju), tid);
-        chunk_store<PF, NT>(pf, lds4, tid);
+        chunk_dma<PF, NT>(rJ, voff, 0, ldsb, wave);
+        wait_vmem();
         __syncthreads();
         uint32_t buf = 0;
         const uint32_t qch = q * CH;
@@ -272,20 +406,20 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
                     if (has_next) {
                         uint32_t ni = last_chunk ? i + 1 : i;
                         uint32_t nc = last_chunk ? ((i + 2) >> 5) : (uint32_t)k + 1;
-                        chunk_load<PF, NT>(
-                            pf, reinterpret_cast<const f4v*>(Ju + ((size_t)ni * nchunk + nc) * chf),
-                            tid);
+                        chunk_dma<PF, NT>(rJ, voff, (ni * nchunk + nc) * chf * 4,
+                                          ldsb + (buf ^ 1) * chf * 4, wave);
                     }
                     const float* cb = lds + buf * chf;
                     const bool diag = (uint32_t)k == kp;
 #pragma unroll
                     for (int s = 0; s < S; s++) {
-                        uint32_t sm = (select_word<KW>(st[s], (uint32_t)k >> 2) >> ((k & 3) * 8)) & 0xffu;
+                        uint32_t sm =
+                            (state_word<KW>(st[s], (uint32_t)k >> 2) >> ((k & 3) * 8)) & 0xffu;
                         float v = cb[base[s] + sm * CH];
                         if (diag) v = (j > owner) ? v : 0.f;
                         acc[s] += v;
                     }
-                    if (has_next) chunk_store<PF, NT>(pf, lds4 + (buf ^ 1) * (chf / 4), tid);
+                    wait_vmem();
                     __syncthreads();
                     buf ^= 1;
                 }

@@ -12,6 +12,7 @@ struct SweepArgs {
     uint32_t ld, nwalk, L, q, nchunk, nsteps, k0, k1, walker_base;
     uint64_t step0;
     float nbeta;
+    uint32_t wpb;  // walkers per work-group: 128, 256 or 512 (0 = default)
 };
 
 struct EnergyArgs {
@@ -21,9 +22,10 @@ struct EnergyArgs {
     uint32_t ld, nseq, L, q, nchunk;
 };
 
-// Walkers per work-group of the sweep kernel is 256 (L <= 256) or 128; the
-// walker count must be a multiple of 256.
-constexpr uint32_t SWEEP_WALKERS_PER_BLOCK = 256;
+// Walkers per work-group of the sweep kernel: 128, 256 or 512 for L <= 256,
+// 128 for longer sequences; the walker count must be a multiple of 512.
+constexpr uint32_t SWEEP_WALKERS_PER_BLOCK = 512;
+constexpr uint32_t SWEEP_DEFAULT_WPB = 512;
 
 // Largest q / L the register-resident kernels are instantiated for.
 constexpr uint32_t MAX_Q = 25;

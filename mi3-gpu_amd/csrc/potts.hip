@@ -20,10 +20,10 @@ hipError_t set_lds(F kernel, size_t bytes) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <int KW, int PF>
-hipError_t sweep_kw(const SweepArgs& a, hipStream_t st) {
-    constexpr uint32_t W = 2 * slots<KW>() * NWAVES;
-    auto kern = k_sweep<KW, slots<KW>(), NWAVES, PF>;
+template <int KW, int PF, int S>
+hipError_t sweep_kws(const SweepArgs& a, hipStream_t st) {
+    constexpr uint32_t W = 2 * S * NWAVES;
+    auto kern = k_sweep<KW, S, NWAVES, PF>;
     size_t lds = 2ull * chunk_floats(a.q) * sizeof(float);
     hipError_t e = set_lds(kern, lds);
     if (e != hipSuccess) return e;
@@ -63,7 +63,16 @@ hipError_t dispatch(const A& a, hipStream_t st, uint32_t nchunk, uint32_t q) {
 
 template <int KW, int PF>
 struct SweepDisp {
-    static hipError_t run(const SweepArgs& a, hipStream_t st) { return sweep_kw<KW, PF>(a, st); }
+    static hipError_t run(const SweepArgs& a, hipStream_t st) {
+        if constexpr (KW <= 2) {
+            uint32_t w = a.wpb ? a.wpb : SWEEP_DEFAULT_WPB;
+            if (w <= 128) return sweep_kws<KW, PF, 8>(a, st);
+            if (w <= 256) return sweep_kws<KW, PF, 16>(a, st);
+            return sweep_kws<KW, PF, 32>(a, st);
+        } else {
+            return sweep_kws<KW, PF, 8>(a, st);
+        }
+    }
 };
 template <int KW, int PF>
 struct EnergyDisp {

@@ -69,6 +69,7 @@ struct mi3_ctx {
     uint32_t indep_calls = 0;
     uint32_t* pos = nullptr;
     size_t pos_cap = 0;
+    uint32_t wpb = 0;  // sweep walkers per work-group (0 = default)
     // derived device data
     std::map<std::string, Chunked> chunked;  // chunked layouts keyed by source buffer
     double* red = nullptr;                    // reduction partials
@@ -422,6 +423,14 @@ int mi3_fill_seqs(mi3_ctx* c, const char* seqbuf, const uint8_t* seq) {
     return MI3_OK;
 }
 
+int mi3_set_wgsize(mi3_ctx* c, uint32_t wgsize) {
+    if (!c) return MI3_EINVAL;
+    if (wgsize == 0 || (wgsize & (wgsize - 1)))
+        return fail(c, MI3_EINVAL, "wgsize must be a power of two");
+    c->wpb = wgsize <= 128 ? 128 : (wgsize <= 256 ? 256 : 512);
+    return MI3_OK;
+}
+
 int mi3_init_mcmc(mi3_ctx* c, uint32_t nsteps, uint64_t rng_offset, uint64_t rng_span) {
     if (!c) return MI3_EINVAL;
     if (c->mcmc) return fail(c, MI3_ESTATE, "Already initialized MCMC");
@@ -468,6 +477,7 @@ int mi3_run_mcmc(mi3_ctx* c, float beta) {
     a.walker_base = c->walker_base;
     a.step0 = c->step;
     a.nbeta = -beta;
+    a.wpb = c->wpb;
     {
         Scope s(c, "mcmc", (size_t)c->N * c->nsteps * (c->L + 1));
         HIPCHK(c, launch_sweep(a, c->stream));

@@ -40,6 +40,7 @@ SIGNATURES = {
     'mi3_buf_fill_u32': (_i, [_P, _S, _u32]),
     'mi3_synchronize': (_i, [_P]),
     'mi3_fill_seqs': (_i, [_P, _S, _P]),
+    'mi3_set_wgsize': (_i, [_P, _u32]),
     'mi3_init_mcmc': (_i, [_P, _u32, _u64, _u64]),
     'mi3_run_mcmc': (_i, [_P, _f]),
     'mi3_mcmc_step_count': (_i, [_P, _c.POINTER(_u64)]),

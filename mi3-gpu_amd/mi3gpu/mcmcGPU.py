@@ -90,8 +90,6 @@ class MCMCGPU:
         self.nhist, self.histws = histogram_heuristic(q)
         if nseq % wgsize != 0:
             raise Exception("nseq per GPU must be a multiple of wgsize")
-        if wgsize < q * q:
-            raise Exception("wgsize cannot be less than q*q")
         self.profile = profile
         self.logfn = Path(outdir) / 'gpu-{}.log'.format(gpunum)
         with open(self.logfn, "wt") as f:
@@ -102,6 +100,7 @@ class MCMCGPU:
                                      int(seed) & 0xFFFFFFFFFFFFFFFF, 1 if profile else 0,
                                      ctypes.byref(ctx)))
         self._ctx = ctx
+        check(self._ctx, lib().mi3_set_wgsize(self._ctx, int(wgsize)))
         self.seed = int(seed)
         self.initted = []
         self.buf_spec = {}
