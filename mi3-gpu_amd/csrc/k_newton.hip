@@ -200,62 +200,89 @@ __global__ void k_add_floats(float* __restrict__ dst, const float* __restrict__ 
 }
 
 // Weighted pair histograms F_{im}(a,b) = sum_n w_n [s_ni = a][s_nm = b]
-// (weightedMarg, mcmc.cl:646-718), unnormalised.  One work-group per
-// (row i, chunk of 32 partner sites m > i): the 32 pair histograms live in
-// LDS as [a*q+b][32] (lane j of each half-wave owns pair (i, 32c+j), so the
-// LDS adds of a half-wave hit 32 distinct banks); every half-wave streams
-// 16 sequences per iteration and the block is written once at the end.
-__global__ __launch_bounds__(256) void k_weighted_marg(const uint8_t* __restrict__ seq,
+// (weightedMarg, mcmc.cl:646-718), unnormalised.
+//
+// Accumulation is exact integer arithmetic: every weight is first converted
+// to 64-bit fixed point, wq_n = round(w_n * 2^62 / sum(w)) (so no bin can
+// exceed 2^62), bins are summed with LDS ds_add_u64 (the LDS unit executes
+// 64-bit integer adds at full rate, float adds ~20x slower), and the result
+// is scaled back to float once.  The rounding error per term is below
+// sum(w) * 2^-63, far under float32 summation error, and the result does
+// not depend on the order of the additions.
+//
+// Pass 1: fixed-point weights (a non-finite or zero total makes every output
+// NaN, which keeps the divergence check of the Newton loop working).
+__global__ void k_weights_fixed(const float* __restrict__ w, uint32_t n,
+                                const float* __restrict__ wsum, uint64_t* __restrict__ wq,
+                                uint32_t* __restrict__ bad) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double tot = (double)wsum[0];
+    const bool ok = tot > 0.0 && tot < INFINITY;
+    if (i == 0) bad[0] = ok ? 0u : 1u;
+    if (i >= n) return;
+    double v = ok ? (double)w[i] * (4611686018427387904.0 / tot) : 0.0;  // 2^62 / sum
+    wq[i] = (v > 0.0) ? (uint64_t)(v + 0.5) : 0ull;
+}
+
+// Pass 2: one work-group per (row i, chunk of 32 partner sites m > i); the
+// 32 pair histograms live in LDS as uint64 [a*q+b][32] (lane j of a
+// half-wave owns pair (i, 32c+j): the adds of a half-wave hit distinct
+// banks).  Each of the 16 half-waves streams 16 sequences per iteration.
+__global__ __launch_bounds__(512) void k_weighted_marg(const uint8_t* __restrict__ seq,
                                                        uint32_t ldseq, uint32_t nseq,
-                                                       const float* __restrict__ w, uint32_t L,
-                                                       uint32_t q, uint32_t nchunk,
+                                                       const uint64_t* __restrict__ wq,
+                                                       const float* __restrict__ wsum,
+                                                       const uint32_t* __restrict__ bad,
+                                                       uint32_t L, uint32_t q,
                                                        const uint32_t* __restrict__ units,
                                                        float* __restrict__ out) {
-    extern __shared__ float hist[];
+    extern __shared__ unsigned long long hist64[];
     const uint32_t unit = units[blockIdx.x];
     const uint32_t i = unit >> 8, c = unit & 0xff;
     const uint32_t qq = q * q;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t e = tid; e < qq * CH; e += 256) hist[e] = 0.f;
+    for (uint32_t e = tid; e < qq * CH; e += 512) hist64[e] = 0ull;
     __syncthreads();
-    const uint32_t lane = tid & 63, j = lane & 31, hw = tid >> 5;  // 8 half-waves
+    const uint32_t lane = tid & 63, j = lane & 31, hw = tid >> 5;  // 16 half-waves
     const uint32_t m = c * CH + j;
     const bool valid = (m > i) && (m < L);
     const uint32_t mrow = valid ? m : i;
     const uint8_t* rowi = seq + (size_t)i * ldseq;
     const uint8_t* rowm = seq + (size_t)mrow * ldseq;
     const uint32_t qch = q * CH;
-    for (uint32_t n0 = hw * 16; n0 < nseq; n0 += 8 * 16) {
+    for (uint32_t n0 = hw * 16; n0 < nseq; n0 += 16 * 16) {
         if (n0 + 16 <= nseq) {
-            uint4 si4 = *reinterpret_cast<const uint4*>(rowi + n0);
-            uint4 sm4 = *reinterpret_cast<const uint4*>(rowm + n0);
-            const float4* w4 = reinterpret_cast<const float4*>(w + n0);
-            float4 wv[4] = {w4[0], w4[1], w4[2], w4[3]};
-            uint32_t si[4] = {si4.x, si4.y, si4.z, si4.w};
-            uint32_t sm[4] = {sm4.x, sm4.y, sm4.z, sm4.w};
-            float ww[16] = {wv[0].x, wv[0].y, wv[0].z, wv[0].w, wv[1].x, wv[1].y, wv[1].z, wv[1].w,
-                            wv[2].x, wv[2].y, wv[2].z, wv[2].w, wv[3].x, wv[3].y, wv[3].z, wv[3].w};
+            const uint4 si4 = *reinterpret_cast<const uint4*>(rowi + n0);
+            const uint4 sm4 = *reinterpret_cast<const uint4*>(rowm + n0);
+            const ulonglong2* w2 = reinterpret_cast<const ulonglong2*>(wq + n0);
+            ulonglong2 wv[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) wv[k] = w2[k];
+            const uint32_t si[4] = {si4.x, si4.y, si4.z, si4.w};
+            const uint32_t sm[4] = {sm4.x, sm4.y, sm4.z, sm4.w};
             if (valid) {
 #pragma unroll
                 for (int e = 0; e < 16; e++) {
-                    uint32_t a = (si[e >> 2] >> (8 * (e & 3))) & 0xffu;
-                    uint32_t b = (sm[e >> 2] >> (8 * (e & 3))) & 0xffu;
-                    atomicAdd(&hist[a * qch + b * CH + j], ww[e]);
+                    const uint32_t a = (si[e >> 2] >> (8 * (e & 3))) & 0xffu;
+                    const uint32_t b = (sm[e >> 2] >> (8 * (e & 3))) & 0xffu;
+                    const unsigned long long v = (e & 1) ? wv[e >> 1].y : wv[e >> 1].x;
+                    atomicAdd(&hist64[__umul24(a, qch) + b * CH + j], v);
                 }
             }
         } else if (valid) {
-            for (uint32_t n = n0; n < nseq && n < n0 + 16; n++) {
-                uint32_t a = rowi[n], b = rowm[n];
-                atomicAdd(&hist[a * qch + b * CH + j], w[n]);
-            }
+            for (uint32_t n = n0; n < nseq && n < n0 + 16; n++)
+                atomicAdd(&hist64[__umul24(rowi[n], qch) + rowm[n] * CH + j],
+                          (unsigned long long)wq[n]);
         }
     }
     __syncthreads();
-    // write the 32 pair blocks (packed order), skipping invalid pairs
-    for (uint32_t e = tid; e < qq * CH; e += 256) {
-        uint32_t jj = e & 31, ac = e >> 5;
-        uint32_t mm = c * CH + jj;
-        if (mm > i && mm < L) out[(size_t)pair_index(i, mm, L) * qq + ac] = hist[e];
+    // back to float: F = acc * sum(w) / 2^62, written once (packed order)
+    const double back = bad[0] ? NAN : (double)wsum[0] * (1.0 / 4611686018427387904.0);
+    for (uint32_t e = tid; e < qq * CH; e += 512) {
+        const uint32_t jj = e & 31, ac = e >> 5;
+        const uint32_t mm = c * CH + jj;
+        if (mm > i && mm < L)
+            out[(size_t)pair_index(i, mm, L) * qq + ac] = (float)((double)hist64[e] * back);
     }
 }
 

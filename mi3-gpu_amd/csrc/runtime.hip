@@ -73,6 +73,10 @@ struct mi3_ctx {
     // derived device data
     std::map<std::string, Chunked> chunked;  // chunked layouts keyed by source buffer
     double* red = nullptr;                    // reduction partials
+    float* red_out = nullptr;                 // 2 floats: internal reduction result
+    uint64_t* wq = nullptr;                   // fixed-point weights
+    size_t wq_cap = 0;
+    uint32_t* wbad = nullptr;
     uint32_t* wm_units = nullptr;
     uint32_t n_units = 0;
     uint8_t* seq1 = nullptr;                  // one sequence (fill_seqs)
@@ -261,6 +265,8 @@ int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t s
     if (hipMalloc(&c->wm_units, units.size() * 4) != hipSuccess ||
         hipMemcpy(c->wm_units, units.data(), units.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&c->red, RED_BLOCKS * 2 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->red_out, 2 * sizeof(float)) != hipSuccess ||
+        hipMalloc(&c->wbad, sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&c->seq1, L) != hipSuccess) {
         mi3_destroy(c);
         return fail(nullptr, MI3_ENOMEM, "device allocation failed");
@@ -284,6 +290,9 @@ int mi3_destroy(mi3_ctx* c) {
     }
     hipFree(c->pos);
     hipFree(c->red);
+    hipFree(c->red_out);
+    hipFree(c->wq);
+    hipFree(c->wbad);
     hipFree(c->wm_units);
     hipFree(c->seq1);
     if (c->t0) hipEventDestroy(c->t0);
@@ -589,13 +598,29 @@ int mi3_weighted_marg(mi3_ctx* c, const char* seqbuf, const char* wbuf, uint32_t
     size_t qq = (size_t)c->q * c->q;
     rc = need(c, "bi", c->nP * qq * 4, &bb);
     if (rc) return rc;
-    size_t lds = qq * CH * 4;
+    if (c->wq_cap < nseq) {
+        hipFree(c->wq);
+        c->wq = nullptr;
+        HIPCHK(c, hipMalloc(&c->wq, (size_t)nseq * sizeof(uint64_t) + 256));
+        c->wq_cap = nseq;
+    }
+    size_t lds = qq * CH * 8;
     HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_weighted_marg),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    Scope s(c, "weightedMarg", (size_t)nseq * (c->L + 4));
-    hipLaunchKernelGGL(k_weighted_marg, dim3(c->n_units), dim3(256), lds, c->stream,
-                       (const uint8_t*)sb->d, (uint32_t)ld, nseq, (const float*)wb->d, c->L, c->q,
-                       c->nchunk, (const uint32_t*)c->wm_units, (float*)bb->d);
+    Scope s(c, "weightedMarg", (size_t)nseq * (c->L + 8));
+    uint32_t nb = grid1(nseq);
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(256), 0, c->stream, (const float*)wb->d,
+                       nseq, 1, c->red);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->stream, (const double*)c->red,
+                       nb, 1, c->red_out);
+    hipLaunchKernelGGL(k_weights_fixed, dim3(grid1(nseq ? nseq : 1)), dim3(256), 0, c->stream,
+                       (const float*)wb->d, nseq, (const float*)c->red_out, c->wq, c->wbad);
+    hipLaunchKernelGGL(k_weighted_marg, dim3(c->n_units), dim3(512), lds, c->stream,
+                       (const uint8_t*)sb->d, (uint32_t)ld, nseq, (const uint64_t*)c->wq,
+                       (const float*)c->red_out, (const uint32_t*)c->wbad, c->L, c->q,
+                       (const uint32_t*)c->wm_units, (float*)bb->d);
     HIPCHK(c, hipGetLastError());
     bb->version++;
     return MI3_OK;
