@@ -10,6 +10,14 @@ site-updates/s over the timed steps, with walker state and couplings resident
 in HBM before timing starts.
 
 Printed as one JSON line, with
+  energy_ess:   energy effective samples per second of the same sampler
+                (walker-averaged integrated autocorrelation time of the
+                per-sweep energy traces, Sokal window c=5) on an equilibrated
+                ensemble;
+  round:        wall seconds per inference round (MCMC of 128 sweeps,
+                bicounts, the quasi-Newton phase and the per-round host work)
+                of `Mi3.py infer` on target bimarginals sampled from the same
+                model, with and without the bz2 sequence write;
   roofline:     algorithmic bytes (L+1 per site update: read the walker's L
                 residues, write back one) per launch over the launch duration
                 measured with HIP events on the engine's stream, against the
@@ -34,6 +42,15 @@ sys.path.insert(0, str(REPO))
 
 METRIC = "site-updates/s & energy-ESS/s at L=232 q=21; s per inference round at fixed SSR"
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per k_sweep launch, measured with two rocprofv3 --pmc runs
+# (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of
+# `bench.py --steps 2 --warmup 1 --no-cpu --ess-sweeps 0 --rounds 0
+# --copy-calibration 1024`, each counter scaled by what it reads for a 1 GiB
+# device-to-device copy in the same run (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
+# summary in profiles/k_sweep_L232_q21_N262144_pmc_traffic.md.  Keyed by
+# (L, q, walkers, sweeps per launch, walkers per work-group).
+MEASURED_TRAFFIC = {(232, 21, 2 ** 18, 8, 512): 13.42e9}
 
 
 def production_model(L=232, q=21, seed=1):
@@ -67,6 +84,98 @@ def cpu_baseline(J, L, q, seed, target_s=12.0):
                       f"{nthr} OpenMP threads)"}
 
 
+ALPHA21 = '-ACDEFGHIKLMNPQRSTVWY'
+
+
+def sample_target(J, L, q, n, sweeps, seed, wgsize):
+    """Sample n sequences of the model (sweeps sweeps from uniform random
+    sequences); returns (seqs (n, L) u8, bimarg with Jeffreys pseudocount)."""
+    from mi3gpu.mcmcGPU import MCMCGPU
+    from mi3gpu.utils.pseudocount import mutation_pc
+    with tempfile.TemporaryDirectory() as d:
+        g = MCMCGPU((0, 0, None, None), L, q, n, wgsize, Path(d), 1024, seed)
+        g.initMCMC(8 * L, 0, 2 ** 63)
+        g.setBuf('J', J)
+        g.setBuf('seq main', np.random.default_rng(seed).integers(0, q, size=(n, L)).astype('u1'))
+        for _ in range(max(1, sweeps // 8)):
+            g.runMCMC()
+        g.calcBicounts('main')
+        bc = g.getBuf('bicount').read()
+        seqs = g.getBuf('seq main').read()
+        del g
+    f = bc.astype(np.float64) / n
+    return seqs, mutation_pc(f, n).astype('<f4')
+
+
+def measure_ess(J, seqs, L, q, T, seed, wgsize):
+    """Energy traces of T single-sweep launches from an equilibrated
+    ensemble; returns the energy-ESS figures."""
+    from mi3gpu.mcmcGPU import MCMCGPU
+    from mi3gpu.utils.ess import energy_ess
+    n = seqs.shape[0]
+    with tempfile.TemporaryDirectory() as d:
+        g = MCMCGPU((0, 0, None, None), L, q, n, wgsize, Path(d), 1024, seed, profile=True)
+        g.initMCMC(L, 0, 2 ** 63)
+        g.setBuf('J', J)
+        g.setBuf('seq main', seqs)
+        g.profile_reset()
+        E = np.empty((T, n), dtype=np.float32)
+        for t in range(T):
+            g.runMCMC()
+            g.calcEnergies('main')
+            E[t] = g.getBuf('E main').read()
+        kms, kcnt = g.kernel_ms('mcmc')
+        del g
+    ess, tau, W = energy_ess(E)
+    t_sample = kms * kcnt * 1e-3
+    half = T // 2
+    return {"value": ess / t_sample, "unit": "energy effective samples/s",
+            "tau_int_sweeps": tau, "window": W, "walkers": n, "sweeps": T,
+            "ess": ess, "sampling_s": t_sample,
+            "mean_E_first_half": float(E[:half].mean()),
+            "mean_E_second_half": float(E[half:].mean())}
+
+
+def measure_rounds(target, L, q, N, rounds, seed):
+    """`Mi3.py infer` rounds at the production size; per-round wall times."""
+    import mi3gpu.Mi3 as Mi3
+    with tempfile.TemporaryDirectory() as d:
+        np.save(Path(d) / 'target.npy', target)
+        args = ['infer', '--init_model', 'independent', '--bimarg', str(Path(d) / 'target.npy'),
+                '--alpha=' + ALPHA21[:q], '--nwalkers', str(N), '--nsteps', str(16 * L),
+                '--equiltime', '8', '--mcsteps', str(rounds), '--rngseed', str(seed),
+                '--outdir', str(Path(d) / 'infer')]
+        import contextlib
+        import io
+        with contextlib.redirect_stdout(io.StringIO()):
+            p = Mi3.main(args)
+        st = p.round_stats
+    floor = float(np.sum(target * (1 - target)) / N)
+    wall = [r['wall_s'] for r in st]
+    nowrite = [r['wall_s'] - r['status_write_s'] for r in st]
+    return {"value": float(np.mean(wall)), "unit": "s/round",
+            "s_per_round_without_seq_write": float(np.mean(nowrite)),
+            "rounds": st, "ssr_floor": floor,
+            "note": "rounds from the independent-site model; SSR column per round"}
+
+
+def copy_calibration(mib):
+    """Plain device-to-device copies of a known size: a profiler run that
+    also covers them turns FETCH_SIZE / WRITE_SIZE counts into bytes."""
+    from mi3gpu._lib import lib, check
+    import ctypes
+    ctx = ctypes.c_void_p()
+    check(None, lib().mi3_create(0, 8, 4, 512, 1, 0, ctypes.byref(ctx)))
+    nbytes = mib << 20
+    for name in (b'copy src', b'copy dst'):
+        check(ctx, lib().mi3_buf_alloc(ctx, name, nbytes))
+    check(ctx, lib().mi3_buf_fill_u32(ctx, b'copy src', 0x01020304))
+    for _ in range(3):
+        check(ctx, lib().mi3_buf_copy(ctx, b'copy dst', b'copy src', nbytes))
+    check(ctx, lib().mi3_synchronize(ctx))
+    lib().mi3_destroy(ctx)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -77,10 +186,20 @@ def main():
     ap.add_argument('--wgsize', type=int, default=512,
                     help="walkers per work-group of the sweep (128, 256, 512)")
     ap.add_argument('--no-cpu', action='store_true', help="skip the CPU baseline")
+    ap.add_argument('--ess-sweeps', type=int, default=128,
+                    help="traced sweeps for energy-ESS (0 = skip)")
+    ap.add_argument('--target-sweeps', type=int, default=256,
+                    help="sweeps to sample the 2^20 target sequences")
+    ap.add_argument('--rounds', type=int, default=2,
+                    help="Mi3 inference rounds to time (0 = skip)")
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help="also run 3 device-to-device copies of MIB MiB (the reference "
+                         "kernel that scales HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
-              "(the path shards by walker; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: this engine "
+              "runs on a single device (the sampler would shard by walker, with one "
+              "all-reduce of the int32 bicounts per round)", file=sys.stderr)
         sys.exit(2)
 
     from mi3gpu.mcmcGPU import MCMCGPU
@@ -107,6 +226,9 @@ def main():
         g.calcEnergies('main')
         E = g.getBuf('E main').read()
 
+    if a.copy_calibration > 0:
+        copy_calibration(a.copy_calibration)
+
     su_per_step = N * nsteps
     value = su_per_step * a.steps / dt
     alg_bytes = su_per_step * (L + 1)
@@ -131,10 +253,18 @@ def main():
                    "walkers_per_workgroup": a.wgsize,
                    "mean_energy_after": float(np.mean(E))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(
+                         (L, q, N, a.sweeps_per_step, a.wgsize)),
                      "kernel": "k_sweep", "kernel_ms": kms, "launches": kcnt,
                      "alg_bytes_per_launch": alg_bytes},
     }
+    if a.ess_sweeps > 0 or a.rounds > 0:
+        tseqs, target = sample_target(J, L, q, 2 ** 20, a.target_sweeps, 4, a.wgsize)
+        if a.ess_sweeps > 0:
+            out["energy_ess"] = measure_ess(J, tseqs[:N], L, q, a.ess_sweeps, 5, a.wgsize)
+        del tseqs
+        if a.rounds > 0:
+            out["round"] = measure_rounds(target, L, q, N, a.rounds, 6)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(J, L, q, seed=3)
     print(json.dumps(out), flush=True)

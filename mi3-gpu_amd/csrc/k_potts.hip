@@ -10,10 +10,11 @@
 //     neighbour sites, laid out [a*q + c][32] so that the 32 lanes of a
 //     half-wave (32 distinct neighbour sites) gather from 32 distinct banks
 //     whatever residues their walkers hold;
-//   * chunks are double-buffered: the next chunk is loaded into registers
-//     while the current one is gathered, one barrier per chunk.
+//   * chunks are double-buffered in LDS: the next chunk is copied from global
+//     memory straight into LDS (buffer_load ... lds) while the current one is
+//     gathered, one barrier per chunk.
 //
-// Chunked coupling layout (built by k_unpack_chunked in k_layout.hip):
+// Chunked coupling layout (built by k_unpack_chunked in k_misc.hip):
 //   Ju[i][c][a*q + b][jl] = J_{i,m}(a, b),  m = 32c + jl,
 // symmetric (both i<m and i>m filled), zero for m == i and m >= L.
 #include "common.h"

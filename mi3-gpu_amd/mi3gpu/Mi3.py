@@ -520,7 +520,7 @@ def inverseIsing(orig_args, infer_args, log):
         log("")
     log("Initialization")
     log("===============")
-    p = attrdict({'outdir': args.outdir})
+    p = attrdict({'outdir': args.outdir, 'round_stats': []})
     setup_seed(args, p, log)
     p.update(process_newton_args(args, log))
     p['L'], p['q'] = getLq(p.bimarg)

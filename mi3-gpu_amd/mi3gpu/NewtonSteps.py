@@ -309,6 +309,7 @@ def MCMCstep(runName, Jstep, couplings, param, gpus, log):
     log("---------------------------")
     log(f"Total J update step {Jstep}")
     log("(Re-zeroing gauge of couplings)")
+    tstart = time.time()
     couplings = fieldlessGaugeEven(np.zeros((L, q)), couplings)[1].astype('<f4')
 
     rundir = outdir / runName
@@ -330,9 +331,11 @@ def MCMCstep(runName, Jstep, couplings, param, gpus, log):
     param.last_mc_per_s = MC_s
 
     seqs = gpus.collect('seq main')
+    tw = time.time()
     writeStatus(runName, Jstep, bimarg_target, bicount, bimarg_model, couplings, seqs, energies,
                 param.alpha, e_rho, ptinfo, outdir, log,
                 write_seqs=not getattr(param, 'skip_seq_write', False))
+    t_write = time.time() - tw
 
     delta = param.newton_delta
     ssr = np.sum((bimarg_target - bimarg_model) ** 2)
@@ -349,6 +352,12 @@ def MCMCstep(runName, Jstep, couplings, param, gpus, log):
     log(f"Increasing newtonsteps to {param.newtonSteps}")
     (outdir / runName / 'nsteps').write_text(str(Jsteps))
     param.last_round_ssr = ssr
+    # per-round timing record (wall s; not part of the reference's outputs)
+    if 'round_stats' in param:
+        param.round_stats.append({'run': runName, 'wall_s': time.time() - tstart,
+                                  'mcmc_s': dt, 'status_write_s': t_write,
+                                  'newton_s': param.last_newton_time, 'newton_steps': Jsteps,
+                                  'equil_launches': equilsteps, 'ssr': float(ssr)})
     return Jstep + Jsteps, seqs, energies, newJ
 
 

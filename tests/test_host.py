@@ -268,3 +268,23 @@ def test_profile_log_summary():
     r = summarize(["EVT mcmc 0 2000000000 8000000000", "EVT mcmc 0 1000000000",
                    "noise", "EVT getBuf 5 10 100"])
     assert r['mcmc'][1] == 2 and abs(r['mcmc'][0] - 3.0) < 1e-9 and r['getBuf'][2] == 100
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.5, 0.8])
+def test_tau_int_ar1_closed_form(phi):
+    """AR(1) x_t = phi x_{t-1} + e_t has rho(t) = phi^t, so the integrated
+    autocorrelation time is (1 + phi) / (2 (1 - phi)); a wrong window or a
+    missing 1/2 misses it by far more than the 5% tolerance."""
+    from mi3gpu.utils.ess import tau_int, energy_ess
+    rng = np.random.default_rng(11)
+    T, n = 2000, 400
+    x = np.zeros((T, n))
+    x[0] = rng.normal(size=n) / np.sqrt(1 - phi ** 2)
+    for t in range(1, T):
+        x[t] = phi * x[t - 1] + rng.normal(size=n)
+    tau, W = tau_int(x)
+    expect = (1 + phi) / (2 * (1 - phi))
+    assert abs(tau - expect) < 0.05 * expect
+    assert W >= 5 * tau
+    ess = energy_ess(x)[0]
+    assert abs(ess - n * T / (2 * expect)) < 0.06 * n * T / (2 * expect)
