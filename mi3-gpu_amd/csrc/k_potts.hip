@@ -195,7 +195,7 @@ __device__ __forceinline__ uint32_t lane_select_u(uint64_t mask, uint32_t a, uin
 // and one transposed butterfly leaves each lane with the dE of its own slot;
 // it decides, and a ballot tells the owner lane of pos which walkers accepted.
 template <int KW, int S, int NWAVES, int PF>
-__global__ __launch_bounds__(64 * NWAVES) void k_sweep(
+__global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_sweep(
     const float* __restrict__ Ju, uint8_t* __restrict__ seq, const uint32_t* __restrict__ pos_list,
     uint32_t ldseq, uint32_t L, uint32_t q, uint32_t nchunk, uint32_t nsteps, uint32_t k0,
     uint32_t k1, uint64_t step0, uint32_t walker_base, float nbeta) {
@@ -285,13 +285,23 @@ __global__ __launch_bounds__(64 * NWAVES) void k_sweep(
                                       wave);
                 }
                 const char* cb = ldsb + buf * chb;
+                // gathers issued in groups of G slots ahead of their use, so
+                // that one LDS wait covers 2*G reads
+                constexpr int G = (S < 8) ? S : 8;
 #pragma unroll
-                for (int s = 0; s < S; s++) {
-                    uint32_t off = ((state_word<KW>(st[s], (uint32_t)k >> 2) >> ((k & 3) * 8)) &
-                                    0xffu) * (CH * 4);
-                    float vA = *reinterpret_cast<const float*>(cb + baseA[s] + off);
-                    float vB = *reinterpret_cast<const float*>(cb + baseB[s] + off);
-                    acc[s] += vB - vA;
+                for (int s0 = 0; s0 < S; s0 += G) {
+                    float vA[G], vB[G];
+#pragma unroll
+                    for (int g = 0; g < G; g++) {
+                        const int s = s0 + g;
+                        uint32_t off =
+                            ((state_word<KW>(st[s], (uint32_t)k >> 2) >> ((k & 3) * 8)) & 0xffu) *
+                            (CH * 4);
+                        vA[g] = *reinterpret_cast<const float*>(cb + baseA[s] + off);
+                        vB[g] = *reinterpret_cast<const float*>(cb + baseB[s] + off);
+                    }
+#pragma unroll
+                    for (int g = 0; g < G; g++) acc[s0 + g] += vB[g] - vA[g];
                 }
                 wait_vmem();
                 __syncthreads();
@@ -411,14 +421,21 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
                                           ldsb + (buf ^ 1) * chf * 4, wave);
                     }
                     const float* cb = lds + buf * chf;
-                    const bool diag = (uint32_t)k == kp;
+                    // on the diagonal chunk only partners m > i count: a 0/1
+                    // factor (fma(v, 1, acc) == acc + v exactly)
+                    const float keep = ((uint32_t)k != kp || j > owner) ? 1.f : 0.f;
+                    constexpr int G = (S < 16) ? S : 16;
 #pragma unroll
-                    for (int s = 0; s < S; s++) {
-                        uint32_t sm =
-                            (state_word<KW>(st[s], (uint32_t)k >> 2) >> ((k & 3) * 8)) & 0xffu;
-                        float v = cb[base[s] + sm * CH];
-                        if (diag) v = (j > owner) ? v : 0.f;
-                        acc[s] += v;
+                    for (int s0 = 0; s0 < S; s0 += G) {
+                        float v[G];
+#pragma unroll
+                        for (int g = 0; g < G; g++) {
+                            uint32_t sm = (state_word<KW>(st[s0 + g], (uint32_t)k >> 2) >>
+                                           ((k & 3) * 8)) & 0xffu;
+                            v[g] = cb[base[s0 + g] + sm * CH];
+                        }
+#pragma unroll
+                        for (int g = 0; g < G; g++) acc[s0 + g] = __fmaf_rn(v[g], keep, acc[s0 + g]);
                     }
                     wait_vmem();
                     __syncthreads();

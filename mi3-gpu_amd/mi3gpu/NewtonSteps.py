@@ -53,15 +53,50 @@ def printstats(name, jstep, bicount, bimarg_target, bimarg_model, couplings, ene
     return disp
 
 
+class _BackgroundWriter:
+    """One worker thread for the per-round sequence files, so that the bzip2
+    compression of N*L bytes overlaps the next GPU phase.  `flush()` waits for
+    every pending write and re-raises the first failure."""
+
+    def __init__(self):
+        self._ex = None
+        self._pending = []
+
+    def submit(self, fn, *args, **kw):
+        from concurrent.futures import ThreadPoolExecutor
+        if self._ex is None:
+            self._ex = ThreadPoolExecutor(max_workers=1)
+        self._pending.append(self._ex.submit(fn, *args, **kw))
+
+    def flush(self):
+        pending, self._pending = self._pending, []
+        for f in pending:
+            f.result()
+
+
+background_writer = _BackgroundWriter()
+
+
+def flush_writes():
+    """Wait for the background sequence writes of earlier rounds."""
+    background_writer.flush()
+
+
 def writeStatus(name, Jstep, bimarg_target, bicount, bimarg_model, couplings, seqs, energies,
-                alpha, e_rho, ptinfo, outdir, log, write_seqs=True):
-    """info.txt + bzip2 seqs of the round (NewtonSteps.py:76-94)."""
+                alpha, e_rho, ptinfo, outdir, log, write_seqs=True, background=True):
+    """info.txt + bzip2 seqs of the round (NewtonSteps.py:76-94).  The seqs
+    file is written by a background thread unless background=False; the
+    array must not be modified afterwards (collect() returns a fresh one)."""
     disp = printstats(name, Jstep, bicount, bimarg_target, bimarg_model, couplings, energies,
                       e_rho, ptinfo)
     with open(outdir / name / 'info.txt', 'wt') as f:
         f.write(disp)
     if write_seqs:
-        writeSeqs(outdir / name / 'seqs', seqs, alpha, zipf=True)
+        if background:
+            background_writer.flush()   # at most one round's file in flight
+            background_writer.submit(writeSeqs, outdir / name / 'seqs', seqs, alpha, zipf=True)
+        else:
+            writeSeqs(outdir / name / 'seqs', seqs, alpha, zipf=True)
     log(disp)
 
 
@@ -412,4 +447,5 @@ def newtonMCMC(param, gpus, start_run, Jstep, log):
             gpus.setSeqs('main', param.seedmsa)
 
         Jstep, seqs, es, J = MCMCstep(runname, Jstep, J, param, gpus, log)
+    flush_writes()
     return J

@@ -105,6 +105,31 @@ def reduceSeqs(fn, reduce_func, startval=None, alpha=prot_alpha):
     return reduce_func(startval, seqs, ids, headers, a), headers
 
 
+def bz2_compress_parallel(data, chunk=8 << 20, nthreads=None):
+    """bzip2-compress `data` as concatenated bzip2 streams, one per ~8 MB
+    piece split at line ends, compressed on a thread pool (the compressor
+    releases the GIL).  Every bzip2 reader (bzip2 -d, Python's bz2.open)
+    decompresses a multi-stream file to the concatenated text, so the file
+    reads back exactly as a single-stream one."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    n = len(data)
+    if n <= chunk:
+        return bz2.compress(data)
+    cuts = [0]
+    while cuts[-1] + chunk < n:
+        nl = data.find(b'\n', cuts[-1] + chunk)
+        if nl < 0:
+            break
+        cuts.append(nl + 1)
+    cuts.append(n)
+    pieces = [data[a:b] for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=nthreads) as ex:
+        return b''.join(ex.map(bz2.compress, pieces))
+
+
 def writeSeqs(fn, seqs, alpha=prot_alpha, ids=None, headers=None, write_param=False,
               zipf=None):
     """Write sequences as text lines, optionally with ids, a PARAM header and
@@ -134,8 +159,8 @@ def writeSeqs(fn, seqs, alpha=prot_alpha, ids=None, headers=None, write_param=Fa
         fn.write(payload)
         return
     if zipf:
-        with bz2.open(fn, 'wb') as f:
-            f.write(payload)
+        with open(fn, 'wb') as f:
+            f.write(bz2_compress_parallel(payload))
     else:
         with open(fn, 'wb') as f:
             f.write(payload)

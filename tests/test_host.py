@@ -288,3 +288,39 @@ def test_tau_int_ar1_closed_form(phi):
     assert W >= 5 * tau
     ess = energy_ess(x)[0]
     assert abs(ess - n * T / (2 * expect)) < 0.06 * n * T / (2 * expect)
+
+
+def test_parallel_bz2_is_standard_multistream(tmp_path):
+    """Multi-stream output decompresses (bz2 module, the stream-by-stream
+    decoder) to the exact text, and the seqs file reads back unchanged."""
+    import bz2
+    from mi3gpu.utils.seqload import bz2_compress_parallel, writeSeqs, loadSeqs
+    rng = np.random.default_rng(4)
+    seqs = rng.integers(0, 21, size=(60000, 232)).astype('u1')   # ~14 MB of text
+    writeSeqs(tmp_path / 's', seqs, zipf=True)
+    raw = (tmp_path / 's').read_bytes()
+    assert raw[:3] == b'BZh' and raw.count(b'BZh91AY&SY') >= 2     # several streams
+    dec = bz2.BZ2Decompressor()
+    first = dec.decompress(raw)
+    assert dec.eof and len(dec.unused_data) > 0                    # really multi-stream
+    text = bz2.decompress(raw)
+    assert text.startswith(first) and text.count(b'\n') == 60000
+    back = loadSeqs(tmp_path / 's')[0]
+    np.testing.assert_array_equal(back, seqs)
+    small = b'ACD\n' * 10
+    assert bz2.decompress(bz2_compress_parallel(small)) == small
+
+
+def test_background_status_write_flushes(tmp_path):
+    import mi3gpu.NewtonSteps as NS
+    from mi3gpu.utils.seqload import loadSeqs
+    L, q = 6, 4
+    nP = L * (L - 1) // 2
+    f = np.full((nP, q * q), 1.0 / (q * q), dtype='f4')
+    seqs = np.random.default_rng(1).integers(0, q, size=(512, L)).astype('u1')
+    (tmp_path / 'run_0').mkdir()
+    NS.writeStatus('run_0', 0, f, None, f, np.zeros_like(f), seqs, np.zeros(512), 'ABCD',
+                   None, None, tmp_path, lambda *a: None)
+    NS.flush_writes()
+    np.testing.assert_array_equal(loadSeqs(tmp_path / 'run_0' / 'seqs', alpha='ABCD')[0], seqs)
+    assert (tmp_path / 'run_0' / 'info.txt').exists()
