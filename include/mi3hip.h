@@ -200,6 +200,29 @@ int mi3_reg_l2z(mi3_ctx *ctx, float gamma, float pc, float lJ);
  * (NewtonSteps.py:218-220). */
 int mi3_update_J_reg(mi3_ctx *ctx, float gamma, float pc, int reg, float lJ);
 
+/* The remaining regularisers of the coupling step, each one pass over the
+ * pair blocks of "dJ" given "J" and "bi" (one q x q block per wave).
+ *   SCADJ   (r, a): SCAD penalty on the zero-gauge couplings
+ *           (reg_SCADJ, mcmcGPU.py:747-759; mcmc.cl:904-932)
+ *   X       (lX): lambda_X |X|, X = sum_ab J(a,b) C(a,b)
+ *           (reg_X, mcmcGPU.py:775-787; kernel mcmc.cl:1004-1028)
+ *   Xij     one lambda per pair read from "Creg" (first nPairs floats,
+ *           as the kernel's lambdas[group]) (reg_Xij, mcmcGPU.py:761-773;
+ *           kernel mcmc.cl:972-1002).  The reference's wrappers call each
+ *           other's kernel with mismatched arguments; here each name runs
+ *           its own kernel's arithmetic.
+ *   SCADX   (s, r, a) (reg_SCADX, mcmcGPU.py:789-801; mcmc.cl:1077-1124)
+ *   expX    (lam) (reg_expX, mcmcGPU.py:803-815; mcmc.cl:1126-1150)
+ *   ddE     (lam) (reg_ddE, mcmcGPU.py:817-829; mcmc.cl:1152-1187)
+ *   SCADddE (lam, r) (reg_SCADddE, mcmcGPU.py:831-843; mcmc.cl:1191-1243) */
+int mi3_reg_SCADJ(mi3_ctx *ctx, float gamma, float pc, float r, float a);
+int mi3_reg_X(mi3_ctx *ctx, float gamma, float pc, float lX);
+int mi3_reg_Xij(mi3_ctx *ctx, float gamma, float pc);
+int mi3_reg_SCADX(mi3_ctx *ctx, float gamma, float pc, float s, float r, float a);
+int mi3_reg_expX(mi3_ctx *ctx, float gamma, float pc, float lam);
+int mi3_reg_ddE(mi3_ctx *ctx, float gamma, float pc, float lam);
+int mi3_reg_SCADddE(mi3_ctx *ctx, float gamma, float pc, float lam, float r);
+
 /* ---- profiling ------------------------------------------------------------ */
 
 /* Append "EVT name start_ns end_ns bytes" lines for every completed launch to

@@ -286,4 +286,214 @@ __global__ __launch_bounds__(512) void k_weighted_marg(const uint8_t* __restrict
     }
 }
 
+// The other regularisers of the coupling step (mcmc.cl:904-1243), one wave
+// per q x q pair block; all read the block Jt = J + dJ and the model
+// bimarginals f of the block and subtract a step from dJ:
+//   SCADJ   SCAD penalty on the zero-gauge J0 (reg_SCADJ 904-932)
+//   X       lambda_X |X| on X = sum Jt*C, C = f - fi fj (reg_X 1004-1028)
+//   Xij     the same with one lambda per pair, lambdas[pair] (reg_Xij 972-1002)
+//   SCADX   SCAD penalty on X (reg_SCADX 1077-1124)
+//   expX    exp(-|X|/lam) penalty (reg_expX 1126-1150)
+//   ddE     sign of every double difference J(a,b)-J(a',b)-J(a,b')+J(a',b')
+//           (reg_ddE 1152-1187)
+//   SCADddE SCAD-weighted double differences (reg_SCADddE 1191-1243)
+// with fi, fj the row and column sums of f and
+//   Xijab(c,d) = sum_ab Jt(a,b) (fi(a) - [a=c]) (fj(b) - [b=d])
+//              = T - U(c) - V(d) + Jt(c,d),
+//   U(c) = sum_b Jt(c,b) fj(b), V(d) = sum_a Jt(a,d) fi(a), T = sum_c fi(c) U(c)
+// (the q^4 double loop of getXijab, mcmc.cl:1052-1067, in q^2 work).
+enum RegMode { REG_SCADJ = 0, REG_X = 1, REG_XIJ = 2, REG_SCADX = 3, REG_EXPX = 4, REG_DDE = 5,
+               REG_SCADDDE = 6 };
+
+__device__ __forceinline__ float wave_sum(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_reg_block(const float* __restrict__ bi,
+                                                   const float* __restrict__ J,
+                                                   float* __restrict__ dJ,
+                                                   const float* __restrict__ lambdas, uint32_t nP,
+                                                   uint32_t q, float gamma, float pc, float p0,
+                                                   float p1, float p2, int mode) {
+    __shared__ float sJ[4][32 * 32];
+    __shared__ float sa[4][2][32];   // row / column quantities
+    const uint32_t wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const uint32_t p = blockIdx.x * 4 + wave;
+    const uint32_t qq = q * q;
+    const bool active = p < nP;   // wave-uniform
+    float jt[16], fv[16], dj[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = l + 64 * r;
+        jt[r] = fv[r] = dj[r] = 0.f;
+        if (active && e < qq) {
+            const size_t o = (size_t)p * qq + e;
+            dj[r] = dJ[o];
+            fv[r] = bi[o];
+            jt[r] = J[o] + dj[r];
+            sJ[wave][e] = jt[r];
+        }
+    }
+    __syncthreads();
+    if (mode == REG_DDE || mode == REG_SCADDDE) {
+        const float lam = p0, rr = p1;
+        if (active) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const uint32_t e = l + 64 * r;
+                if (e < qq) {
+                    const uint32_t a = e / q, b = e - a * q;
+                    float dR = 0.f;
+                    for (uint32_t g = 1; g < q; g++) {
+                        const uint32_t a2 = (a + g) % q;
+                        const float jr = sJ[wave][e] - sJ[wave][a2 * q + b];
+                        for (uint32_t d = 1; d < q; d++) {
+                            const uint32_t b2 = (b + d) % q;
+                            const float ddE = jr + (sJ[wave][a2 * q + b2] - sJ[wave][a * q + b2]);
+                            const float sg = fsign(ddE);
+                            if (mode == REG_DDE) {
+                                dR += sg;
+                            } else {
+                                const float ad = fabsf(ddE);
+                                const float R = ad < rr ? 1.f
+                                                : (ad < 4.f * rr ? (4.f * rr - ad) / (4.f * rr - rr)
+                                                                 : 0.f);
+                                dR += lam * sg * R;
+                            }
+                        }
+                    }
+                    dR = dR / (float)((q - 1) * (q - 1));
+                    const float step = (mode == REG_DDE) ? lam * dR * gamma / (fv[r] + pc)
+                                                         : gamma * dR / (fv[r] + pc);
+                    dJ[(size_t)p * qq + e] = dj[r] - step;
+                }
+            }
+        }
+        return;
+    }
+    if (mode == REG_SCADJ) {
+        // zero-gauge J0 of Jt: row means, column means, total mean
+        if (active) {
+            if (l < q) {
+                float s = 0.f;
+                for (uint32_t b = 0; b < q; b++) s += sJ[wave][l * q + b];
+                sa[wave][0][l] = s / q;
+            } else if (l >= 32 && l - 32 < q) {
+                const uint32_t b = l - 32;
+                float s = 0.f;
+                for (uint32_t a = 0; a < q; a++) s += sJ[wave][a * q + b];
+                sa[wave][1][b] = s / q;
+            }
+        }
+        __syncthreads();
+        if (!active) return;
+        float mean = 0.f;
+        for (uint32_t a = 0; a < q; a++) mean += sa[wave][0][a];
+        mean /= q;
+        const float lJ = p0, aa = p1;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const uint32_t e = l + 64 * r;
+            if (e < qq) {
+                const uint32_t a = e / q, b = e - a * q;
+                const float hi = sa[wave][0][a] - mean, hj = sa[wave][1][b] - mean;
+                const float J0 = jt[r] - (hi + hj) - mean;
+                const float aJ = fabsf(J0);
+                float R = 0.f;
+                if (aJ < lJ)
+                    R = fminf(lJ, aJ / gamma);
+                else if (aJ < aa * lJ)
+                    R = (aa * lJ - aJ) / (aa - 1.f);
+                R *= gamma * fsign(J0) / (fv[r] + pc);
+                dJ[(size_t)p * qq + e] = dj[r] - R;
+            }
+        }
+        return;
+    }
+    // X family: unimarginals of the block's bimarginals
+    if (active) {
+        if (l < q) {
+            float s = 0.f;
+            for (uint32_t b = 0; b < q; b++) s += bi[(size_t)p * qq + l * q + b];
+            sa[wave][0][l] = s;
+        } else if (l >= 32 && l - 32 < q) {
+            const uint32_t b = l - 32;
+            float s = 0.f;
+            for (uint32_t a = 0; a < q; a++) s += bi[(size_t)p * qq + a * q + b];
+            sa[wave][1][b] = s;
+        }
+    }
+    __syncthreads();
+    float C[16];
+    float xs = 0.f, xn = 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = l + 64 * r;
+        C[r] = 0.f;
+        if (active && e < qq) {
+            const uint32_t a = e / q, b = e - a * q;
+            C[r] = fv[r] - sa[wave][0][a] * sa[wave][1][b];
+            xs += jt[r] * C[r];
+            xn += C[r] * C[r] / (fv[r] + pc);
+        }
+    }
+    const float X = wave_sum(xs), Xnorm = wave_sum(xn);
+    const bool needXab = (mode == REG_SCADX || mode == REG_EXPX);
+    __shared__ float sUV[4][2][32];
+    float T = 0.f;
+    if (needXab) {
+        if (active) {
+            if (l < q) {
+                float u = 0.f;
+                for (uint32_t b = 0; b < q; b++) u += sJ[wave][l * q + b] * sa[wave][1][b];
+                sUV[wave][0][l] = u;
+            } else if (l >= 32 && l - 32 < q) {
+                const uint32_t d = l - 32;
+                float v = 0.f;
+                for (uint32_t a = 0; a < q; a++) v += sJ[wave][a * q + d] * sa[wave][0][a];
+                sUV[wave][1][d] = v;
+            }
+        }
+        __syncthreads();
+        if (active)
+            for (uint32_t c = 0; c < q; c++) T += sa[wave][0][c] * sUV[wave][0][c];
+    }
+    if (!active) return;
+    const float sX = fsign(X), aX = fabsf(X);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t e = l + 64 * r;
+        if (e >= qq) continue;
+        const uint32_t a = e / q, b = e - a * q;
+        const float f = fv[r];
+        float out;
+        if (mode == REG_X || mode == REG_XIJ) {
+            const float lX0 = (mode == REG_X) ? p0 : lambdas[p];
+            const float lX = fminf(lX0, aX / (Xnorm * gamma));
+            out = (mode == REG_X) ? dj[r] - gamma * lX * C[r] * sX / (f + pc)
+                                  : dj[r] - gamma * (lX * C[r] * sX) / (f + pc);
+        } else {
+            const float Xab = T - sUV[wave][0][a] - sUV[wave][1][b] + jt[r];
+            const float grad = C[r] + f * (X - Xab);
+            if (mode == REG_SCADX) {
+                const float s = p0, rr0 = p1, aa = p2;
+                float R = 0.f;
+                if (aX < rr0) {
+                    const float rr = fminf(rr0, aX / (Xnorm * gamma * s));
+                    R = rr * sX;
+                } else if (aX < aa * rr0) {
+                    R = (aa * rr0 * sX - X) / (aa - 1.f);
+                }
+                R *= gamma * s * grad / (f + pc);
+                out = dj[r] - R;
+            } else {  // expX
+                const float R = sX * expf(-aX / p0) * grad;
+                out = dj[r] - gamma * R / (f + pc);
+            }
+        }
+        dJ[(size_t)p * qq + e] = out;
+    }
+}
+
 }  // namespace mi3

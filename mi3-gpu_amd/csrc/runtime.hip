@@ -757,6 +757,65 @@ int mi3_update_J_reg(mi3_ctx* c, float gamma, float pc, int reg, float lJ) {
     return update_reg(c, "dJ", gamma, pc, lJ, reg == 0 ? 0 : reg + 2, "updateJ_reg");
 }
 
+static int reg_block(mi3_ctx* c, int mode, float gamma, float pc, float p0, float p1, float p2,
+                     const char* label) {
+    Buf *bi, *J, *dJ, *cr = nullptr;
+    size_t n = (size_t)c->nP * c->q * c->q * 4;
+    int rc = need(c, "bi", n, &bi);
+    if (rc) return rc;
+    rc = need(c, "J", n, &J);
+    if (rc) return rc;
+    rc = need(c, "dJ", n, &dJ);
+    if (rc) return rc;
+    if (mode == REG_XIJ) {
+        rc = need(c, "Creg", (size_t)c->nP * 4, &cr);
+        if (rc) return rc;
+    }
+    Scope s(c, label, n * 4);
+    hipLaunchKernelGGL(k_reg_block, dim3((c->nP + 3) / 4), dim3(256), 0, c->stream,
+                       (const float*)bi->d, (const float*)J->d, (float*)dJ->d,
+                       cr ? (const float*)cr->d : (const float*)nullptr, c->nP, c->q, gamma, pc,
+                       p0, p1, p2, mode);
+    HIPCHK(c, hipGetLastError());
+    dJ->version++;
+    return MI3_OK;
+}
+
+int mi3_reg_SCADJ(mi3_ctx* c, float gamma, float pc, float r, float a) {
+    if (!c) return MI3_EINVAL;
+    return reg_block(c, REG_SCADJ, gamma, pc, r, a, 0.f, "reg_SCADJ");
+}
+
+int mi3_reg_X(mi3_ctx* c, float gamma, float pc, float lX) {
+    if (!c) return MI3_EINVAL;
+    return reg_block(c, REG_X, gamma, pc, lX, 0.f, 0.f, "reg_X");
+}
+
+int mi3_reg_Xij(mi3_ctx* c, float gamma, float pc) {
+    if (!c) return MI3_EINVAL;
+    return reg_block(c, REG_XIJ, gamma, pc, 0.f, 0.f, 0.f, "reg_Xij");
+}
+
+int mi3_reg_SCADX(mi3_ctx* c, float gamma, float pc, float s, float r, float a) {
+    if (!c) return MI3_EINVAL;
+    return reg_block(c, REG_SCADX, gamma, pc, s, r, a, "reg_SCADX");
+}
+
+int mi3_reg_expX(mi3_ctx* c, float gamma, float pc, float lam) {
+    if (!c) return MI3_EINVAL;
+    return reg_block(c, REG_EXPX, gamma, pc, lam, 0.f, 0.f, "reg_expX");
+}
+
+int mi3_reg_ddE(mi3_ctx* c, float gamma, float pc, float lam) {
+    if (!c) return MI3_EINVAL;
+    return reg_block(c, REG_DDE, gamma, pc, lam, 0.f, 0.f, "reg_ddE");
+}
+
+int mi3_reg_SCADddE(mi3_ctx* c, float gamma, float pc, float lam, float r) {
+    if (!c) return MI3_EINVAL;
+    return reg_block(c, REG_SCADDDE, gamma, pc, lam, r, 0.f, "reg_SCADddE");
+}
+
 int mi3_profile_flush(mi3_ctx* c, const char* path) {
     if (!c) return MI3_EINVAL;
     FILE* f = path ? fopen(path, "a") : nullptr;

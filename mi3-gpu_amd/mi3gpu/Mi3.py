@@ -254,7 +254,80 @@ def process_GPU_args(args, L, q, outdir, log):
 
 
 REG_TYPES = ['l2z', 'l1z', 'SCADJ', 'X', 'Xij', 'SCADX', 'expX', 'ddE', 'SCADddE']
-SUPPORTED_REG = ['l1z', 'l2z']
+
+
+def parse_reg(spec, bimarg, log):
+    """--reg TYPE:ARGS -> {'reg': TYPE, 'regarg': tuple or array}, with the
+    argument formats and defaults of the reference (Mi3.py:1038-1113)."""
+    rtype, _, rarg = spec.partition(':')
+    if rtype not in REG_TYPES:
+        raise Exception(f"reg must be one of {str(REG_TYPES)}")
+    if rtype == 'ddE':
+        lam = float(rarg)
+        log(f"Regularizing using {rtype} with lambda = {lam}")
+        return {'reg': rtype, 'regarg': (lam,)}
+    if rtype == 'SCADddE':
+        lam, _, r = rarg.partition(':')
+        lam = float(lam)
+        r = lam if r == '' else float(r)
+        log(f"Regularizing using {rtype} with lambda = {lam}, r = {r}")
+        return {'reg': rtype, 'regarg': (lam, r)}
+    if rtype in ('l2z', 'l1z'):
+        try:
+            lJ = float(rarg)
+        except ValueError:
+            raise Exception(f"{rtype} specifier must be of form '{rtype}:lJ', eg "
+                            f"'{rtype}:0.01'. Got '{spec}'")
+        log(f"Regularizing using {rtype} norm with lambda_J = {lJ}")
+        return {'reg': rtype, 'regarg': (lJ,)}
+    if rtype == 'SCADJ':
+        try:
+            r, _, a = rarg.partition(':')
+            r = float(r)
+            a = float(a) if a != '' else 4.0
+        except ValueError:
+            raise Exception(f"{rtype} specifier must be of form '{rtype}:r:a', eg "
+                            f"'{rtype}:10:0.1'. Got '{spec}'")
+        if a < 2.0:
+            raise Exception("SCADJ a parameter must be >= 2.0")
+        log(f"Regularizing using SCADJ with r={r} a={a}")
+        return {'reg': rtype, 'regarg': (r, a)}
+    if rtype == 'X':
+        try:
+            lX = float(rarg)
+        except ValueError:
+            raise Exception(f"{rtype} specifier must be of form 'X:lX', eg 'X:0.01'. "
+                            f"Got '{spec}'")
+        log(f"Regularizing X with lambda_X = {lX}")
+        return {'reg': rtype, 'regarg': (lX,)}
+    if rtype == 'Xij':
+        log(f"Regularizing with Xij from file {rarg}")
+        lam = np.load(rarg)
+        if lam.shape != bimarg.shape:
+            raise Exception("Xij in wrong format")
+        return {'reg': rtype, 'regarg': lam.astype('<f4')}
+    if rtype == 'SCADX':
+        try:
+            d, _, r = rarg.partition(':')
+            r, _, a = r.partition(':')
+            d, r = float(d), float(r)
+            a = float(a) if a != '' else 4.0
+        except ValueError:
+            raise Exception(f"{rtype} specifier must be of form '{rtype}:d:r:a' or "
+                            f"{rtype}:d:r', eg '{rtype}:10:0.1'. Got '{spec}'")
+        if a < 2.0:
+            raise Exception("SCADX a parameter must be >= 2.0")
+        s = 2 * d / ((1 + a) * r * r)   # cost d for large |X|
+        log(f"Regularizing using SCADX with d={d} r={r} a={a}")
+        return {'reg': rtype, 'regarg': (s, r, a)}
+    # expX
+    try:
+        lam = float(rarg)
+    except ValueError:
+        raise Exception(f"{rtype} specifier must be of form 'expX:l', eg 'expX:0.001'. "
+                        f"Got '{spec}'")
+    log(f"Regularizing using expX with  lam = {lam}")
+    return {'reg': rtype, 'regarg': (lam,)}
 
 
 def process_newton_args(args, log):
@@ -279,20 +352,7 @@ def process_newton_args(args, log):
     log("Target Marginals: " + printsome(bimarg) + "...")
     p['bimarg'] = bimarg
     if args.reg is not None:
-        rtype, _, rarg = args.reg.partition(':')
-        if rtype not in REG_TYPES:
-            raise Exception(f"reg must be one of {str(REG_TYPES)}")
-        if rtype not in SUPPORTED_REG:
-            raise Exception(f"reg '{rtype}' is not part of this build (supported: "
-                            f"{SUPPORTED_REG})")
-        try:
-            lJ = float(rarg)
-        except ValueError:
-            raise Exception(f"{rtype} specifier must be of form '{rtype}:lJ', eg "
-                            f"'{rtype}:0.01'. Got '{args.reg}'")
-        log(f"Regularizing using {rtype} norm with lambda_J = {lJ}")
-        p['reg'] = rtype
-        p['regarg'] = (lJ,)
+        p.update(parse_reg(args.reg, bimarg, log))
     log("")
     return p
 

@@ -161,7 +161,9 @@ def iterNewton(param, bimarg_model, gpus, log):
     for i in range(newtonSteps):
         gpus.updateJ(gamma, pc)
         if param.reg is not None:
-            gpus.reg(param.reg, (gamma, pc) + tuple(param.regarg))
+            # Xij takes its lambdas from the 'Creg' buffer set once per run
+            extra = () if param.reg == 'Xij' else tuple(param.regarg)
+            gpus.reg(param.reg, (gamma, pc) + extra)
         gpus.calcEnergies(seqbuf, 'dJ')
         if param.beta is not None:
             gpus.addFloatBuf(ebuf, etmp)

@@ -59,6 +59,13 @@ SIGNATURES = {
     'mi3_reg_l1z': (_i, [_P, _f, _f, _f]),
     'mi3_reg_l2z': (_i, [_P, _f, _f, _f]),
     'mi3_update_J_reg': (_i, [_P, _f, _f, _i, _f]),
+    'mi3_reg_SCADJ': (_i, [_P, _f, _f, _f, _f]),
+    'mi3_reg_X': (_i, [_P, _f, _f, _f]),
+    'mi3_reg_Xij': (_i, [_P, _f, _f]),
+    'mi3_reg_SCADX': (_i, [_P, _f, _f, _f, _f, _f]),
+    'mi3_reg_expX': (_i, [_P, _f, _f, _f]),
+    'mi3_reg_ddE': (_i, [_P, _f, _f, _f]),
+    'mi3_reg_SCADddE': (_i, [_P, _f, _f, _f, _f]),
     'mi3_profile_flush': (_i, [_P, _S]),
     'mi3_profile_kernel_ms': (_i, [_P, _S, _c.POINTER(_c.c_double), _c.POINTER(_i)]),
     'mi3_profile_reset': (_i, [_P]),

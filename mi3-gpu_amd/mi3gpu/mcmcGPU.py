@@ -380,12 +380,55 @@ class MCMCGPU:
         self.log("reg_l2z")
         self._reg(2, gamma, pc, lJ)
 
-    def _unsupported_reg(self, *a, **k):
-        raise NotImplementedError("this regularizer is not part of this build "
-                                  "(supported: l1z, l2z)")
+    @_pending_flush
+    def reg_SCADJ(self, gamma, pc, r, a, wait_for=None):
+        """SCAD penalty on the zero-gauge couplings (mcmcGPU.py:747-759)."""
+        self.require('Jstep')
+        self.log("reg_SCADJ")
+        check(self._ctx, lib().mi3_reg_SCADJ(self._ctx, float(gamma), float(pc), float(r),
+                                             float(a)))
 
-    reg_SCADJ = reg_Xij = reg_X = reg_SCADX = reg_expX = reg_ddE = reg_SCADddE = \
-        _unsupported_reg
+    @_pending_flush
+    def reg_X(self, gamma, pc, lX, wait_for=None):
+        """lambda_X |X| penalty, X = sum J*C (mcmcGPU.py:775-787, kernel
+        mcmc.cl:1004-1028)."""
+        self.require('Jstep')
+        self.log("reg X")
+        check(self._ctx, lib().mi3_reg_X(self._ctx, float(gamma), float(pc), float(lX)))
+
+    @_pending_flush
+    def reg_Xij(self, gamma, pc, wait_for=None):
+        """Per-pair lambda_X read from the 'Creg' buffer (mcmcGPU.py:761-773,
+        kernel mcmc.cl:972-1002)."""
+        self.require('Jstep')
+        self.log("reg Xij")
+        check(self._ctx, lib().mi3_reg_Xij(self._ctx, float(gamma), float(pc)))
+
+    @_pending_flush
+    def reg_SCADX(self, gamma, pc, s, r, a, wait_for=None):
+        self.require('Jstep')
+        self.log("reg_SCADX")
+        check(self._ctx, lib().mi3_reg_SCADX(self._ctx, float(gamma), float(pc), float(s),
+                                             float(r), float(a)))
+
+    @_pending_flush
+    def reg_expX(self, gamma, pc, lam, wait_for=None):
+        self.require('Jstep')
+        self.log("reg_expX")
+        check(self._ctx, lib().mi3_reg_expX(self._ctx, float(gamma), float(pc), float(lam)))
+
+    @_pending_flush
+    def reg_ddE(self, gamma, pc, lam, wait_for=None):
+        self.require('Jstep')
+        self.log("reg ddE")
+        check(self._ctx, lib().mi3_reg_ddE(self._ctx, float(gamma), float(pc), float(lam)))
+
+    @_pending_flush
+    def reg_SCADddE(self, gamma, pc, lam, r, wait_for=None):
+        self.require('Jstep')
+        self.log("reg SCADddE")
+        check(self._ctx, lib().mi3_reg_SCADddE(self._ctx, float(gamma), float(pc), float(lam),
+                                               float(r)))
 
     @_pending_flush
     def getBuf(self, bufname, truncateLarge=True, wait_for=None):

@@ -245,6 +245,143 @@ def reg_l2z(bimarg, gamma, pc, lJ, J, dJ):
     return _reg_z(bimarg, gamma, pc, lJ, J, dJ, 'l2')
 
 
+def _blocks(x, q):
+    return np.asarray(x, dtype=np.float64).reshape(-1, q, q)
+
+
+def block_unimarg(bimarg, q):
+    """Row sums fi(a) = sum_b f(a,b) and column sums fj(b) of every pair
+    block (getUnimarg, mcmc.cl:934-970)."""
+    B = _blocks(bimarg, q)
+    return B.sum(axis=2), B.sum(axis=1)
+
+
+def block_X(bimarg, Jt, q, pc):
+    """Per block: C = f - fi fj, X = sum Jt*C, Xnorm = sum C^2/(f + pc)
+    (reg_X / reg_Xij / reg_SCADX / reg_expX, mcmc.cl:972-1150)."""
+    F = _blocks(bimarg, q)
+    T = _blocks(Jt, q)
+    fi, fj = F.sum(axis=2), F.sum(axis=1)
+    C = F - fi[:, :, None] * fj[:, None, :]
+    X = np.sum(T * C, axis=(1, 2))
+    Xn = np.sum(C * C / (F + pc), axis=(1, 2))
+    return C, X, Xn, fi, fj
+
+
+def block_Xijab(Jt, fi, fj, q):
+    """Xijab(c,d) = sum_ab Jt(a,b) (fi(a) - [a=c]) (fj(b) - [b=d])
+    (getXijab, mcmc.cl:1052-1067), as one contraction per block."""
+    T = _blocks(Jt, q)
+    I = np.eye(q)
+    Fi = fi[:, :, None] - I[None]          # [p, a, c]
+    Fj = fj[:, :, None] - I[None]          # [p, b, d]
+    return np.einsum('pab,pac,pbd->pcd', T, Fi, Fj)
+
+
+def reg_SCADJ(bimarg, gamma, pc, lJ, a, J, dJ):
+    """SCAD step on the zero-gauge J0 of J+dJ (reg_SCADJ, mcmc.cl:904-932):
+    R = min(lJ, |J0|/gamma) for |J0| < lJ, (a lJ - |J0|)/(a-1) for
+    |J0| < a lJ, else 0; dJ -= R gamma sign(J0)/(f + pc).  R is continuous
+    in J0, so there is no ambiguous branch."""
+    J, dJ = np.asarray(J, np.float64), np.asarray(dJ, np.float64)
+    bi = np.asarray(bimarg, np.float64)
+    J0 = zero_gauge_blocks(J + dJ)
+    aJ = np.abs(J0)
+    R = np.where(aJ < lJ, np.minimum(lJ, aJ / gamma),
+                 np.where(aJ < a * lJ, (a * lJ - aJ) / (a - 1), 0.0))
+    return dJ - R * gamma * np.sign(J0) / (bi + pc)
+
+
+def reg_X(bimarg, gamma, pc, lX, J, dJ, lambdas=None):
+    """X penalty (reg_X, mcmc.cl:1004-1028; with per-pair lambdas, reg_Xij
+    972-1002): lX' = min(lX, |X|/(Xnorm gamma)),
+    dJ -= gamma lX' C sign(X)/(f + pc)."""
+    J, dJ = np.asarray(J, np.float64), np.asarray(dJ, np.float64)
+    bi = np.asarray(bimarg, np.float64)
+    nP, qq = bi.shape
+    q = int(round(np.sqrt(qq)))
+    C, X, Xn, _, _ = block_X(bi, J + dJ, q, pc)
+    lam = np.full(nP, float(lX)) if lambdas is None else np.asarray(lambdas, np.float64)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        lXe = np.fmin(lam, np.abs(X) / (Xn * gamma))
+    R = (lXe * np.sign(X))[:, None] * C.reshape(nP, qq)
+    return dJ - gamma * R / (bi + pc)
+
+
+def reg_SCADX(bimarg, gamma, pc, s, r, a, J, dJ):
+    """SCAD penalty on X (reg_SCADX, mcmc.cl:1077-1124).  Continuous in X."""
+    J, dJ = np.asarray(J, np.float64), np.asarray(dJ, np.float64)
+    bi = np.asarray(bimarg, np.float64)
+    nP, qq = bi.shape
+    q = int(round(np.sqrt(qq)))
+    Jt = J + dJ
+    C, X, Xn, fi, fj = block_X(bi, Jt, q, pc)
+    Xab = block_Xijab(Jt, fi, fj, q).reshape(nP, qq)
+    aX = np.abs(X)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        rr = np.fmin(r, aX / (Xn * gamma * s))
+    R = np.where(aX < r, rr * np.sign(X),
+                 np.where(aX < a * r, (a * r * np.sign(X) - X) / (a - 1), 0.0))
+    grad = C.reshape(nP, qq) + bi * (X[:, None] - Xab)
+    return dJ - R[:, None] * gamma * s * grad / (bi + pc)
+
+
+def reg_expX(bimarg, gamma, pc, lam, J, dJ):
+    """exp(-|X|/lam) penalty (reg_expX, mcmc.cl:1126-1150).  Returns
+    (new dJ, ambiguous mask): blocks with X ~ 0, where sign(X) jumps."""
+    J, dJ = np.asarray(J, np.float64), np.asarray(dJ, np.float64)
+    bi = np.asarray(bimarg, np.float64)
+    nP, qq = bi.shape
+    q = int(round(np.sqrt(qq)))
+    Jt = J + dJ
+    C, X, Xn, fi, fj = block_X(bi, Jt, q, pc)
+    Xab = block_Xijab(Jt, fi, fj, q).reshape(nP, qq)
+    grad = C.reshape(nP, qq) + bi * (X[:, None] - Xab)
+    R = (np.sign(X) * np.exp(-np.abs(X) / lam))[:, None] * grad
+    scale = np.sum(np.abs(Jt.reshape(nP, qq) * C.reshape(nP, qq)), axis=1)
+    amb = np.repeat((np.abs(X) <= 1e-5 * scale)[:, None], qq, axis=1)
+    return dJ - gamma * R / (bi + pc), amb
+
+
+def _double_diffs(Jt, q):
+    """dd[p, a, b, a2, b2] = Jt(a,b) - Jt(a2,b) - Jt(a,b2) + Jt(a2,b2)."""
+    T = _blocks(Jt, q)
+    a_b = T[:, :, :, None, None]                          # Jt(a, b)
+    a2_b = T.transpose(0, 2, 1)[:, None, :, :, None]       # Jt(a2, b)
+    a_b2 = T[:, :, None, None, :]                         # Jt(a, b2)
+    a2_b2 = T[:, None, None, :, :]                        # Jt(a2, b2)
+    return a_b - a2_b - a_b2 + a2_b2
+
+
+def reg_ddE(bimarg, gamma, pc, lam, J, dJ, scad_r=None):
+    """Double-difference penalty (reg_ddE, mcmc.cl:1152-1187; with scad_r,
+    reg_SCADddE 1191-1243): over all a2 != a, b2 != b,
+      ddE = Jt(a,b) - Jt(a2,b) - Jt(a,b2) + Jt(a2,b2),
+      ddE:     dR = sum sign(ddE) / (q-1)^2,       dJ -= lam dR gamma/(f+pc)
+      SCADddE: dR = sum lam sign(ddE) R(|ddE|) / (q-1)^2, dJ -= gamma dR/(f+pc)
+    with R = 1 below r, (4r - |ddE|)/(3r) below 4r, else 0.  Returns
+    (new dJ, ambiguous mask): entries with some |ddE| ~ 0 (sign jump)."""
+    J, dJ = np.asarray(J, np.float64), np.asarray(dJ, np.float64)
+    bi = np.asarray(bimarg, np.float64)
+    nP, qq = bi.shape
+    q = int(round(np.sqrt(qq)))
+    dd = _double_diffs(J + dJ, q)                       # [p, a, b, a2, b2]
+    off = (1 - np.eye(q))[:, None, :, None] * (1 - np.eye(q))[None, :, None, :]
+    sg = np.sign(dd) * off[None]
+    if scad_r is None:
+        dR = sg.sum(axis=(3, 4)) / (q - 1) ** 2
+        step = lam * dR.reshape(nP, qq) * gamma / (bi + pc)
+    else:
+        r = scad_r
+        ad = np.abs(dd)
+        R = np.where(ad < r, 1.0, np.where(ad < 4 * r, (4 * r - ad) / (4 * r - r), 0.0))
+        dR = (lam * sg * R).sum(axis=(3, 4)) / (q - 1) ** 2
+        step = gamma * dR.reshape(nP, qq) / (bi + pc)
+    scale = np.abs(_blocks(J + dJ, q)).max(axis=(1, 2))[:, None, None, None, None]
+    amb = ((np.abs(dd) <= 1e-5 * scale) & (off[None] > 0)).any(axis=(3, 4)).reshape(nP, qq)
+    return dJ - step, amb
+
+
 def min_floats(x):
     """min(0, min x): the accumulator of minFloats starts at 0 (mcmc.cl:568)."""
     return min(0.0, float(np.min(x)))

@@ -273,3 +273,61 @@ def test_iter_newton_matches_oracle(gpu_ok, tmp_path, reg):
     assert i == oi
     np.testing.assert_allclose(trialJ, oJ, rtol=1e-4, atol=1e-6)
     np.testing.assert_allclose(bim, obim, rtol=1e-4, atol=1e-7)
+
+
+REG_CASES = [
+    ('SCADJ', (0.05, 3.0)),
+    ('X', (0.01,)),
+    ('X', (1e3,)),          # clipped at |X|/(Xnorm gamma) everywhere
+    ('Xij', None),
+    ('SCADX', (1.0, 0.002, 3.0)),
+    ('expX', (0.002,)),
+    ('ddE', (0.01,)),
+    ('SCADddE', (0.01, 0.05)),
+]
+
+
+@pytest.mark.parametrize("L,q", [(12, 21), (20, 4)])
+@pytest.mark.parametrize("reg,arg", REG_CASES)
+def test_other_regularisers_match_oracle(gpu_ok, tmp_path, L, q, reg, arg):
+    gamma, pc = 4e-4, 1e-3
+    g, J, s, bt, bi, dJ = _newton_setup(tmp_path, L, q, 512)
+    nP = J.shape[0]
+    lam = None
+    if reg == 'Xij':
+        creg = np.zeros_like(J)
+        lam = np.random.default_rng(9).uniform(0.001, 0.05, nP).astype('<f4')
+        creg.ravel()[:nP] = lam       # the kernel reads lambdas[pair]
+        g.setBuf('Creg', creg)
+        g.reg_Xij(gamma, pc)
+    else:
+        getattr(g, 'reg_' + reg)(gamma, pc, *arg)
+    got = g.getBuf('dJ').read().astype(np.float64)
+    d0 = dJ.astype(np.float64)
+    amb = np.zeros(J.shape, bool)
+    if reg == 'SCADJ':
+        want = po.reg_SCADJ(bi, gamma, pc, *arg, J, d0)
+    elif reg == 'X':
+        want = po.reg_X(bi, gamma, pc, arg[0], J, d0)
+    elif reg == 'Xij':
+        want = po.reg_X(bi, gamma, pc, None, J, d0, lambdas=lam)
+    elif reg == 'SCADX':
+        want = po.reg_SCADX(bi, gamma, pc, *arg, J, d0)
+    elif reg == 'expX':
+        want, amb = po.reg_expX(bi, gamma, pc, arg[0], J, d0)
+    elif reg == 'ddE':
+        want, amb = po.reg_ddE(bi, gamma, pc, arg[0], J, d0)
+    else:
+        want, amb = po.reg_ddE(bi, gamma, pc, arg[0], J, d0, scad_r=arg[1])
+    ok = ~amb
+    step_w, step_g = want - d0, got - d0
+    assert np.abs(step_w).max() > 0, "the case takes no step"
+    bscale = np.maximum(np.abs(want), np.abs(d0)).max(axis=1, keepdims=True) * np.ones_like(want)
+    sscale = np.abs(step_w).max(axis=1, keepdims=True) * np.ones_like(want) + 1e-30
+    err = (np.abs(got - want) / bscale)[ok]
+    serr = (np.abs(step_g - step_w) / np.maximum(sscale, 1e-7 * bscale))[ok]
+    print(f"{reg}{arg}: max err {err.max():.3g} (block scale), step err {serr.max():.3g}, "
+          f"ambiguous {amb.mean():.4f}")
+    assert err.max() < 1e-5
+    assert serr.max() < 2e-3
+    assert amb.mean() < 0.02

@@ -215,9 +215,26 @@ def test_newton_args_validation(tmp_path):
     a = process_newton_args(p.parse_args(['--bimarg', str(tmp_path / 'b.npy'), '--reg',
                                           'l1z:0.0001']), lambda *a: None)
     assert a.reg == 'l1z' and a.regarg == (0.0001,)
-    with pytest.raises(Exception, match="not part of this build"):
-        process_newton_args(p.parse_args(['--bimarg', str(tmp_path / 'b.npy'), '--reg',
-                                          'SCADJ:1']), lambda *a: None)
+    def reg(spec):
+        return process_newton_args(p.parse_args(['--bimarg', str(tmp_path / 'b.npy'), '--reg',
+                                                 spec]), lambda *a: None)
+    # argument formats and defaults of the reference's option parser
+    assert reg('SCADJ:1').regarg == (1.0, 4.0)
+    assert reg('SCADJ:1:2.5').regarg == (1.0, 2.5)
+    with pytest.raises(Exception, match=">= 2.0"):
+        reg('SCADJ:1:1.5')
+    assert reg('X:0.01').regarg == (0.01,)
+    s, r, a_ = reg('SCADX:10:0.1').regarg
+    assert (r, a_) == (0.1, 4.0) and abs(s - 2 * 10 / (5 * 0.01)) < 1e-9
+    assert reg('expX:0.001').regarg == (0.001,)
+    assert reg('ddE:0.5').regarg == (0.5,)
+    assert reg('SCADddE:0.5').regarg == (0.5, 0.5)
+    assert reg('SCADddE:0.5:0.2').regarg == (0.5, 0.2)
+    np.save(tmp_path / 'creg.npy', np.full(b.shape, 0.01, 'f4'))
+    assert reg('Xij:' + str(tmp_path / 'creg.npy')).regarg.shape == b.shape
+    np.save(tmp_path / 'bad.npy', np.zeros(3, 'f4'))
+    with pytest.raises(Exception, match="wrong format"):
+        reg('Xij:' + str(tmp_path / 'bad.npy'))
     with pytest.raises(Exception, match="must be one of"):
         process_newton_args(p.parse_args(['--bimarg', str(tmp_path / 'b.npy'), '--reg',
                                           'bogus:1']), lambda *a: None)

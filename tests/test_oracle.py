@@ -340,3 +340,103 @@ def test_jeffreys_pseudocount():
     out = po.jeffreys_pseudocount(b, 10)
     np.testing.assert_allclose(out.sum(axis=1), 1)
     assert out.min() > b.min()
+
+
+def test_Xijab_contraction_matches_quadruple_loop():
+    """getXijab's literal q^4 loop (mcmc.cl:1052-1067) on random blocks."""
+    rng = np.random.default_rng(21)
+    q, nP = 3, 4
+    Jt = rng.normal(size=(nP, q * q))
+    fi, fj = rng.random((nP, q)), rng.random((nP, q))
+    got = po.block_Xijab(Jt, fi, fj, q)
+    for p in range(nP):
+        B = Jt[p].reshape(q, q)
+        for c in range(q):
+            for d in range(q):
+                x = 0.0
+                for a in range(q):
+                    for b in range(q):
+                        x += B[a, b] * (fi[p, a] - (a == c)) * (fj[p, b] - (b == d))
+                assert abs(got[p, c, d] - x) < 1e-12
+
+
+def _normalised_blocks(rng, nP, q):
+    f = rng.random((nP, q * q)) + 0.05
+    return f / f.sum(axis=1, keepdims=True)
+
+
+def test_X_family_hand_example():
+    """q=2: f = [[.4,.1],[.1,.4]] -> fi = fj = (.5,.5), C = f - .25,
+    J = [[1,0],[0,0]] -> X = 0.15, Xnorm = 0.5625 (pc = 0)."""
+    f = np.array([[0.4, 0.1, 0.1, 0.4]])
+    J = np.array([[1.0, 0.0, 0.0, 0.0]])
+    dJ = np.zeros_like(J)
+    C, X, Xn, fi, fj = po.block_X(f, J, 2, 0.0)
+    assert abs(X[0] - 0.15) < 1e-12 and abs(Xn[0] - 0.5625) < 1e-12
+    # reg_X, lambda 1: min(1, 0.15/(0.5625*0.01)) = 1 -> dJ = -0.01*C/f
+    got = po.reg_X(f, 0.01, 0.0, 1.0, J, dJ)
+    np.testing.assert_allclose(got, -0.01 * np.array([[0.375, -1.5, -1.5, 0.375]]), rtol=1e-12)
+    # lambda above the clip: lX' = |X|/(Xnorm*gamma) = 26.666..
+    got = po.reg_X(f, 0.01, 0.0, 100.0, J, dJ)
+    np.testing.assert_allclose(got, -0.01 * (0.15 / 0.005625) * np.array(
+        [[0.15 / 0.4, -1.5, -1.5, 0.15 / 0.4]]), rtol=1e-12)
+    # Xijab = (.5-[c=0])(.5-[d=0]); grad = C + f (X - Xijab) = +-0.11
+    np.testing.assert_allclose(po.block_Xijab(J, fi, fj, 2)[0],
+                               [[0.25, -0.25], [-0.25, 0.25]], atol=1e-15)
+    got, amb = po.reg_expX(f, 0.01, 0.0, 1.0, J, dJ)
+    grad = np.array([0.11, -0.11, -0.11, 0.11])
+    np.testing.assert_allclose(got[0], -0.01 * np.exp(-0.15) * grad / f[0], rtol=1e-12)
+    assert not amb.any()
+
+
+def test_X_family_is_gauge_invariant():
+    """Adding u(a) + v(b) to a block leaves X and X - Xijab unchanged when
+    the block's bimarginals sum to 1, so every X-type step is unchanged."""
+    rng = np.random.default_rng(5)
+    q, nP = 4, 6
+    f = _normalised_blocks(rng, nP, q)
+    J = rng.normal(0, 0.3, (nP, q * q))
+    dJ = rng.normal(0, 0.05, (nP, q * q))
+    g = (rng.normal(size=(nP, q, 1)) + rng.normal(size=(nP, 1, q))).reshape(nP, q * q)
+    args = (1e-3, 1e-3)
+    pairs = [(po.reg_X(f, *args, 0.5, J, dJ), po.reg_X(f, *args, 0.5, J + g, dJ)),
+             (po.reg_SCADX(f, *args, 2.0, 0.3, 3.0, J, dJ),
+              po.reg_SCADX(f, *args, 2.0, 0.3, 3.0, J + g, dJ)),
+             (po.reg_expX(f, *args, 0.2, J, dJ)[0], po.reg_expX(f, *args, 0.2, J + g, dJ)[0])]
+    for a, b in pairs:
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)
+
+
+def test_SCADJ_piecewise_by_hand():
+    """Zero-gauge 2x2 block [[x,-x],[-x,x]], f = 1/4, pc = 0, gamma = .1,
+    r = .5, a = 3: R = min(r, |x|/gamma), (a r - |x|)/(a-1), or 0, and the
+    step is R gamma sign(x)/f."""
+    f = np.full((1, 4), 0.25)
+    for x, R in [(0.01, 0.1), (0.04, 0.4), (1.0, 0.25), (-1.2, 0.15), (2.0, 0.0)]:
+        J = np.array([[x, -x, -x, x]])
+        got = po.reg_SCADJ(f, 0.1, 0.0, 0.5, 3.0, J, np.zeros_like(J))
+        exp = -R * 0.1 * np.sign(J) / 0.25
+        np.testing.assert_allclose(got, exp, rtol=1e-12, atol=1e-15)
+
+
+def test_ddE_gauge_invariance_and_q2_closed_form():
+    rng = np.random.default_rng(8)
+    q, nP = 3, 5
+    f = _normalised_blocks(rng, nP, q)
+    J = rng.normal(0, 0.3, (nP, q * q))
+    dJ = np.zeros_like(J)
+    g = (rng.normal(size=(nP, q, 1)) + rng.normal(size=(nP, 1, q))).reshape(nP, q * q)
+    a, amb_a = po.reg_ddE(f, 0.01, 1e-3, 0.5, J, dJ)
+    b, amb_b = po.reg_ddE(f, 0.01, 1e-3, 0.5, J + g, dJ)
+    np.testing.assert_allclose(a, b, atol=1e-12)
+    # q = 2: one double difference per entry, dR = sign(ddE)
+    J2 = np.array([[0.3, -0.1, 0.2, 0.5]])
+    dd = 0.3 - 0.2 - (-0.1) + 0.5     # J(a,b) - J(a',b) - J(a,b') + J(a',b'), a=b=0
+    f2 = np.full((1, 4), 0.25)
+    out, amb = po.reg_ddE(f2, 0.1, 0.0, 2.0, J2, np.zeros_like(J2))
+    assert abs(out[0, 0] - (-2.0 * np.sign(dd) * 0.1 / 0.25)) < 1e-12
+    # SCADddE below r: R = 1 -> same as lam * sign; above 4r: nothing
+    out2, _ = po.reg_ddE(f2, 0.1, 0.0, 2.0, J2, np.zeros_like(J2), scad_r=10.0)
+    np.testing.assert_allclose(out2, out, atol=1e-12)
+    out3, _ = po.reg_ddE(f2, 0.1, 0.0, 2.0, J2, np.zeros_like(J2), scad_r=0.01)
+    np.testing.assert_allclose(out3, 0.0, atol=1e-15)
