@@ -73,7 +73,7 @@ def cpu_baseline(J, L, q, seed, target_s=12.0):
     t0 = time.perf_counter()
     s = cboracle.metropolis_f32(s, J, seed, 0, L, nthreads=nthr)
     t1 = time.perf_counter() - t0
-    sweeps = int(max(1, min(64, target_s / max(t1, 1e-3))))
+    sweeps = int(max(1, min(4096, target_s / max(t1, 1e-3))))
     t0 = time.perf_counter()
     cboracle.metropolis_f32(s, J, seed, 2 * L, sweeps * L, nthreads=nthr)
     dt = time.perf_counter() - t0

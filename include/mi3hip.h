@@ -110,6 +110,18 @@ int mi3_synchronize(mi3_ctx *ctx);
  * (fillSeqs, mcmcGPU.py:927-932). */
 int mi3_fill_seqs(mi3_ctx *ctx, const char *seqbuf, const uint8_t *seq);
 
+/* Copy sequences [src_offset, src_offset+n) of one seq buffer to
+ * [dst_offset, dst_offset+n) of another (storeSeqs / restoreSeqs,
+ * mcmcGPU.py:934-998; mcmc.cl:67-96). */
+int mi3_copy_seqs(mi3_ctx *ctx, const char *src, uint32_t src_offset, const char *dst,
+                  uint32_t dst_offset, uint32_t n);
+
+/* At every site i with marks[i] != 0 set residue i of every sequence of dst
+ * to residue i of sequence `seqind` of src (copySubseq, mcmcGPU.py:1002-1014;
+ * mcmc.cl:98-122).  `marks` names a buffer of at least L bytes. */
+int mi3_copy_subseq(mi3_ctx *ctx, const char *src, uint32_t seqind, const char *dst,
+                    const char *marks);
+
 /* ---- MCMC ---------------------------------------------------------------- */
 
 /* Launch shape of the sweep: walkers per work-group (128, 256 or 512; other

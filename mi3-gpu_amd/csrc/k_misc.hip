@@ -81,4 +81,15 @@ __global__ void k_fill_seqs(uint8_t* __restrict__ seq, uint32_t ldseq, uint32_t 
     seq[(size_t)i * ldseq + w] = s[i];
 }
 
+// Subsequence substitution (copySubseq, mcmc.cl:98-122): every sequence of
+// dst takes, at each marked site, the residue of sequence `seqind` of src.
+__global__ void k_copy_subseq(uint8_t* __restrict__ dst, uint32_t lddst, uint32_t ndst,
+                              const uint8_t* __restrict__ src, uint32_t ldsrc, uint32_t seqind,
+                              const uint8_t* __restrict__ marks, uint32_t L) {
+    uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint64_t)L * ndst) return;
+    uint32_t i = (uint32_t)(idx / ndst), w = (uint32_t)(idx - (uint64_t)i * ndst);
+    if (marks[i]) dst[(size_t)i * lddst + w] = src[(size_t)i * ldsrc + seqind];
+}
+
 }  // namespace mi3

@@ -432,6 +432,47 @@ int mi3_fill_seqs(mi3_ctx* c, const char* seqbuf, const uint8_t* seq) {
     return MI3_OK;
 }
 
+int mi3_copy_seqs(mi3_ctx* c, const char* src, uint32_t src_offset, const char* dst,
+                  uint32_t dst_offset, uint32_t n) {
+    if (!c) return MI3_EINVAL;
+    Buf *bs, *bd;
+    int rc = need(c, src, (size_t)c->L, &bs);
+    if (rc) return rc;
+    rc = need(c, dst, (size_t)c->L, &bd);
+    if (rc) return rc;
+    const size_t lds = seq_ld(c, bs), ldd = seq_ld(c, bd);
+    if ((size_t)src_offset + n > lds || (size_t)dst_offset + n > ldd)
+        return fail(c, MI3_EINVAL, "sequence range past the end of a seq buffer");
+    if (n == 0) return MI3_OK;
+    Scope s(c, "copySeqs", (size_t)2 * n * c->L);
+    HIPCHK(c, hipMemcpy2DAsync((uint8_t*)bd->d + dst_offset, ldd, (const uint8_t*)bs->d + src_offset,
+                               lds, n, c->L, hipMemcpyDeviceToDevice, c->stream));
+    bd->version++;
+    return MI3_OK;
+}
+
+int mi3_copy_subseq(mi3_ctx* c, const char* src, uint32_t seqind, const char* dst,
+                    const char* marks) {
+    if (!c) return MI3_EINVAL;
+    Buf *bs, *bd, *bm;
+    int rc = need(c, src, (size_t)c->L, &bs);
+    if (rc) return rc;
+    rc = need(c, dst, (size_t)c->L, &bd);
+    if (rc) return rc;
+    rc = need(c, marks, (size_t)c->L, &bm);
+    if (rc) return rc;
+    const size_t lds = seq_ld(c, bs), ldd = seq_ld(c, bd);
+    if (seqind >= lds) return fail(c, MI3_EINVAL, "given index is past end of the seq buffer");
+    uint64_t n = (uint64_t)c->L * ldd;
+    Scope s(c, "copySubseq", n);
+    hipLaunchKernelGGL(k_copy_subseq, dim3(grid1(n)), dim3(256), 0, c->stream, (uint8_t*)bd->d,
+                       (uint32_t)ldd, (uint32_t)ldd, (const uint8_t*)bs->d, (uint32_t)lds, seqind,
+                       (const uint8_t*)bm->d, c->L);
+    HIPCHK(c, hipGetLastError());
+    bd->version++;
+    return MI3_OK;
+}
+
 int mi3_set_wgsize(mi3_ctx* c, uint32_t wgsize) {
     if (!c) return MI3_EINVAL;
     if (wgsize == 0 || (wgsize & (wgsize - 1)))

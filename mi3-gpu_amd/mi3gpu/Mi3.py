@@ -1,13 +1,12 @@
 #!/usr/bin/env python3
-"""Mi3 command line: `Mi3.py {infer,gen,energies,benchmark} [options]`
+"""Mi3 command line: `Mi3.py {infer,gen,energies,benchmark,subseq} [options]`
 (Mi3.py of the reference).
 
 Same modes, option names, defaults, log lines and output directory layout.
 Option parsing uses argparse with the reference's `--config` file support
 (configargparse's "key = value" format, read and written) built in, because
-configargparse is not available.  MPI (`--mpi`) and the `subseq` mode are not
-part of this build; `benchmark` works (the reference's crashes at
-Mi3.py:667).
+configargparse is not available.  MPI (`--mpi`) is not part of this build;
+`benchmark` works (the reference's crashes at Mi3.py:667).
 """
 import atexit
 import datetime
@@ -803,7 +802,95 @@ def equilibrate(orig_args, args, log):
 
 
 def subseqFreq(orig_args, args, log):
-    raise Exception("the subseq mode is not part of this build")
+    """`subseq`: log relative frequency of subsequences at fixed positions
+    against a background sample (Mi3.py:872-976).  For every subsequence
+    sequence n, the fixed positions of every background sequence are
+    replaced by those of n and log f_n = logsumexp(E_orig - E_substituted)
+    (or, with --iterbackgrounds, the roles of the two sets swap)."""
+    from scipy.special import logsumexp
+    parser = Parser(progname + ' subseqFreq',
+                    'Compute relative frequency of subsequences at fixed positions')
+    parser.add_argument('fixpos', help="comma separated list of fixed positions")
+    parser.add_argument('--iterbackgrounds', action='store_true',
+                        help='use if backgrounds is small')
+    parser.addopt('GPU options', 'nsteps wgsize gpus profile beta')
+    parser.addopt('Potts Model Options', 'alpha couplings L')
+    parser.addopt(None, 'outdir')
+    parser.add_argument('backgroundseqs', help="large sample of equilibrated sequences")
+    parser.add_argument('subseqs', help="sequences from which to compute subseq freqs")
+    args = parser.parse_args(args)
+    args.measurefperror = False
+    args.outdir.mkdir(parents=True)
+    logfile, log = _open_log(args.outdir / 'log')
+    print_node_startup(log, orig_args)
+    log("Initialization")
+    log("===============")
+    log("")
+    p = attrdict({'outdir': args.outdir})
+    args.trackequil = 0
+    args.init_model = None
+    p.update(process_potts_args(args, getattr(args, 'L', None), None, None, log))
+    L, q, alpha = p.L, p.q, p.alpha
+
+    bseqs = loadSequenceFile(args.backgroundseqs, alpha, log)
+    sseqs = loadSequenceFile(args.subseqs, alpha, log)
+    large, small = (sseqs, bseqs) if args.iterbackgrounds else (bseqs, sseqs)
+    ns = len(small)
+    if ns < 512 or ns % 512 != 0:
+        n_new = ns - 1 + 512 - ((ns - 1) % 512)
+        small = np.concatenate([small, np.zeros((n_new - ns, L), dtype='u1')], axis=0)
+    args.nwalkers = small.shape[0]
+    p.update(process_GPU_args(args, L, q, p.outdir, log))
+    p['rngseed'] = 0
+    gpus = setup_GPUs(p, log, splitwalkers=False)
+    gpus.initMCMC(p.nsteps)
+    gpulseqs = np.array_split(large, gpus.ngpus)
+    gpus.initLargeBufs(gpulseqs[0].shape[0])
+    gpus.initSubseq()
+
+    fixedpos = np.array([int(x) for x in args.fixpos.split(',')])
+    if args.iterbackgrounds:
+        members = set(fixedpos)
+        fixedpos = np.array([i for i in range(L) if i not in members])
+    log(fixedpos)
+    fixedmarks = np.zeros(L, dtype='u1')
+    fixedmarks[fixedpos] = 1
+
+    gpus.setBuf('seq main', small)
+    gpus.setBuf('seq large', gpulseqs)
+    gpus.markPos(fixedmarks)
+    gpus.setBuf('J', p.couplings)
+    log("")
+    log("Subsequence Frequency Calculation")
+    log("=================================")
+    log("")
+    gpus.calcEnergies('main')
+    origEs = gpus.collect('E main')[:ns]
+    log("Getting substituted energies...")
+    if args.iterbackgrounds:
+        logf = np.full(len(large), -np.inf)
+        for n, eo in enumerate(origEs):
+            gpus.copySubseq(n)
+            gpus.calcEnergies('large')
+            energies = gpus.collect('E large')
+            np.logaddexp(logf, eo - energies, out=logf)
+    else:
+        # per subsequence n: log sum_b exp(E_b - E_b[n's fixed positions]) over
+        # the background sequences b.  (The reference subtracts the
+        # subsequence set's energies here, which only broadcasts when both
+        # sets have the same size; the background energies are meant.)
+        gpus.calcEnergies('large')
+        backEs = gpus.collect('E large')
+        logf = np.zeros(ns)
+        for n in range(ns):
+            gpus.copySubseq(n)
+            gpus.calcEnergies('large')
+            energies = gpus.collect('E large')
+            logf[n] = logsumexp(backEs - energies)
+    log("Done! Saving result (log frequency) to file.")
+    np.save(p.outdir / 'probs.npy', logf)
+    logfile.close()
+    return logf
 
 
 def main(args):

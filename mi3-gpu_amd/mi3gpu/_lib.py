@@ -59,6 +59,8 @@ SIGNATURES = {
     'mi3_reg_l1z': (_i, [_P, _f, _f, _f]),
     'mi3_reg_l2z': (_i, [_P, _f, _f, _f]),
     'mi3_update_J_reg': (_i, [_P, _f, _f, _i, _f]),
+    'mi3_copy_seqs': (_i, [_P, _S, _u32, _S, _u32, _u32]),
+    'mi3_copy_subseq': (_i, [_P, _S, _u32, _S, _S]),
     'mi3_reg_SCADJ': (_i, [_P, _f, _f, _f, _f]),
     'mi3_reg_X': (_i, [_P, _f, _f, _f]),
     'mi3_reg_Xij': (_i, [_P, _f, _f]),

@@ -520,35 +520,64 @@ class MCMCGPU:
             raise ValueError("seed sequence must have length L")
         check(self._ctx, lib().mi3_fill_seqs(self._ctx, enc('seq ' + seqbufname), as_ptr(s)))
 
+    @_pending_flush
     def storeSeqs(self, seqs=None, wait_for=None):
-        """Append sequences (or the main buffer) to the large buffer
-        (mcmcGPU.py:934-978)."""
+        """Append sequences (or the whole main buffer) to the large buffer
+        (mcmcGPU.py:934-978), as one device-side copy."""
         self.require('Large')
         offset = self.nstoredseqs
+        self.log("storeSeqs " + str(offset))
         if seqs is None:
-            seqs = self.getBuf('seq main').read()
-        nseq, L = seqs.shape
-        if L != self.L:
-            raise Exception("Sequences have wrong length: {} vs {}".format(L, self.L))
-        if offset + nseq > self.nseq['large']:
-            raise Exception("cannot store seqs past end of large buffer")
-        cur = self.getBuf('seq large', truncateLarge=True).read() if offset else \
-            np.zeros((0, self.L), dtype='<u1')
-        self.setBuf('seq large', np.concatenate([cur, seqs.astype('<u1')]))
+            nseq = self.nseq['main']
+            if offset + nseq > self.nseq['large']:
+                raise Exception("cannot store seqs past end of large buffer")
+            src = 'seq main'
+        else:
+            seqs = np.asarray(seqs)
+            nseq, L = seqs.shape
+            if L != self.L:
+                raise Exception("Sequences have wrong length: {} vs {}".format(L, self.L))
+            if offset + nseq > self.nseq['large']:
+                raise Exception("cannot store seqs past end of large buffer")
+            if seqs.dtype != np.dtype('u1'):
+                raise ValueError("sequences must be uint8")
+            if seqs.size and int(seqs.max()) >= self.q:
+                raise ValueError("sequence residues must be < q = {}".format(self.q))
+            if not lib().mi3_buf_exists(self._ctx, b'seq stage'):
+                check(self._ctx, lib().mi3_buf_alloc(self._ctx, b'seq stage',
+                                                     self.L * self._large_ld))
+            dev = self._site_major(seqs, self._large_ld)
+            check(self._ctx, lib().mi3_buf_write(self._ctx, b'seq stage', as_ptr(dev),
+                                                 dev.nbytes, 0))
+            src = 'seq stage'
+        check(self._ctx, lib().mi3_copy_seqs(self._ctx, enc(src), 0, b'seq large', offset,
+                                             nseq))
+        self.nstoredseqs += nseq
 
     def clearLargeSeqs(self):
         self.require('Large')
         self.nstoredseqs = 0
 
+    @_pending_flush
     def restoreSeqs(self, wait_for=None):
+        """Main buffer <- the first nseq stored large-buffer sequences (the
+        reference's restoreSeqs reads an undefined offset; this one uses 0)."""
         self.require('Large')
-        if self.nstoredseqs < self.nseq['main']:
+        nseq = self.nseq['main']
+        if self.nstoredseqs < nseq:
             raise Exception("not enough seqs stored in large buffer")
-        s = self.getBuf('seq large').read()[:self.nseq['main']]
-        self.setBuf('seq main', s)
+        check(self._ctx, lib().mi3_copy_seqs(self._ctx, b'seq large', 0, b'seq main', 0, nseq))
 
+    @_pending_flush
     def copySubseq(self, seqind, wait_for=None):
-        raise NotImplementedError("subsequence mode is not part of this build")
+        """Every large-buffer sequence takes main sequence `seqind`'s
+        residues at the marked positions (mcmcGPU.py:1002-1014)."""
+        self.require('Subseq')
+        self.log("copySubseq " + str(seqind))
+        if seqind >= self.nseq['main']:
+            raise Exception("given index is past end of main seq buffer")
+        check(self._ctx, lib().mi3_copy_subseq(self._ctx, b'seq main', int(seqind),
+                                               b'seq large', b'markpos'))
 
     def wait(self):
         self._flush_update()

@@ -281,15 +281,18 @@ def block_Xijab(Jt, fi, fj, q):
 def reg_SCADJ(bimarg, gamma, pc, lJ, a, J, dJ):
     """SCAD step on the zero-gauge J0 of J+dJ (reg_SCADJ, mcmc.cl:904-932):
     R = min(lJ, |J0|/gamma) for |J0| < lJ, (a lJ - |J0|)/(a-1) for
-    |J0| < a lJ, else 0; dJ -= R gamma sign(J0)/(f + pc).  R is continuous
-    in J0, so there is no ambiguous branch."""
+    |J0| < a lJ, else 0; dJ -= R gamma sign(J0)/(f + pc).  The step is
+    continuous in J0, but for |J0| < lJ gamma it equals J0/(f + pc), which
+    magnifies the float32 rounding of J0 by 1/(f + pc): those entries are
+    returned as ambiguous.  Returns (new dJ, ambiguous mask)."""
     J, dJ = np.asarray(J, np.float64), np.asarray(dJ, np.float64)
     bi = np.asarray(bimarg, np.float64)
     J0 = zero_gauge_blocks(J + dJ)
     aJ = np.abs(J0)
     R = np.where(aJ < lJ, np.minimum(lJ, aJ / gamma),
                  np.where(aJ < a * lJ, (a * lJ - aJ) / (a - 1), 0.0))
-    return dJ - R * gamma * np.sign(J0) / (bi + pc)
+    amb = aJ < 1.01 * lJ * gamma
+    return dJ - R * gamma * np.sign(J0) / (bi + pc), amb
 
 
 def reg_X(bimarg, gamma, pc, lX, J, dJ, lambdas=None):

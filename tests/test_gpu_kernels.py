@@ -306,7 +306,7 @@ def test_other_regularisers_match_oracle(gpu_ok, tmp_path, L, q, reg, arg):
     d0 = dJ.astype(np.float64)
     amb = np.zeros(J.shape, bool)
     if reg == 'SCADJ':
-        want = po.reg_SCADJ(bi, gamma, pc, *arg, J, d0)
+        want, amb = po.reg_SCADJ(bi, gamma, pc, *arg, J, d0)
     elif reg == 'X':
         want = po.reg_X(bi, gamma, pc, arg[0], J, d0)
     elif reg == 'Xij':
@@ -325,9 +325,44 @@ def test_other_regularisers_match_oracle(gpu_ok, tmp_path, L, q, reg, arg):
     bscale = np.maximum(np.abs(want), np.abs(d0)).max(axis=1, keepdims=True) * np.ones_like(want)
     sscale = np.abs(step_w).max(axis=1, keepdims=True) * np.ones_like(want) + 1e-30
     err = (np.abs(got - want) / bscale)[ok]
-    serr = (np.abs(step_g - step_w) / np.maximum(sscale, 1e-7 * bscale))[ok]
+    # the step relative to the block's largest step, beyond the float32
+    # resolution of the stored dJ (4 ulp of |dJ|)
+    ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+    serr = (np.maximum(np.abs(step_g - step_w) - 4 * ulp, 0) / sscale)[ok]
     print(f"{reg}{arg}: max err {err.max():.3g} (block scale), step err {serr.max():.3g}, "
           f"ambiguous {amb.mean():.4f}")
     assert err.max() < 1e-5
     assert serr.max() < 2e-3
     assert amb.mean() < 0.02
+
+
+def test_store_restore_and_copy_subseq(gpu_ok, tmp_path):
+    """storeSeqs (device and host sources), restoreSeqs and copySubseq on the
+    site-major buffers, against numpy."""
+    L, q, N = 37, 5, 512
+    g = make_gpu(tmp_path, L, q, N)
+    g.initLargeBufs(3 * N + 100)
+    g.initSubseq()
+    a = seqs_rand(N, L, q, 1)
+    b = seqs_rand(N, L, q, 2)
+    g.setBuf('seq main', a)
+    g.storeSeqs()                        # device copy of main
+    g.storeSeqs(b)                       # host sequences
+    g.setBuf('seq main', b)
+    g.storeSeqs()
+    big = g.getBuf('seq large').read()
+    assert big.shape == (3 * N, L)
+    np.testing.assert_array_equal(big, np.concatenate([a, b, b]))
+    with pytest.raises(Exception, match="past end"):
+        g.storeSeqs()
+    g.setBuf('seq main', seqs_rand(N, L, q, 3))
+    g.restoreSeqs()
+    np.testing.assert_array_equal(g.getBuf('seq main').read(), a)
+    marks = np.zeros(L, 'u1')
+    marks[[0, 7, 36]] = 1
+    g.markPos(marks)
+    g.copySubseq(5)
+    big2 = g.getBuf('seq large').read()
+    want = big.copy()
+    want[:, [0, 7, 36]] = a[5, [0, 7, 36]]
+    np.testing.assert_array_equal(big2, want)

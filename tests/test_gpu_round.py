@@ -127,3 +127,43 @@ def test_gen_energies_benchmark_modes(gpu_ok, tmp_path):
                      '--seedseq', 'ABCD' * 5, '--outdir', str(tmp_path / 'bench')])
     assert rate > 0
     assert 'MC steps per second' in (tmp_path / 'bench' / 'log').read_text()
+
+
+@pytest.mark.parametrize("iterback", [False, True])
+def test_subseq_mode(gpu_ok, tmp_path, iterback):
+    """`Mi3.py subseq`: log relative subsequence frequencies from
+    substituted energies, against the fp64 energies oracle."""
+    import mi3gpu.Mi3 as Mi3
+    from mi3gpu.utils.seqload import writeSeqs
+    from scipy.special import logsumexp
+    L, q = 10, 4
+    rng = np.random.default_rng(6)
+    J = rng.normal(0, 0.3, size=(L * (L - 1) // 2, q * q)).astype('f4')
+    back = rng.integers(0, q, size=(700, L)).astype('u1')
+    subs = rng.integers(0, q, size=(40, L)).astype('u1')
+    np.save(tmp_path / 'J.npy', J)
+    writeSeqs(tmp_path / 'back', back, alpha='ABCD')
+    writeSeqs(tmp_path / 'subs', subs, alpha='ABCD')
+    fix = [2, 5, 9]
+    args = ['subseq', ','.join(map(str, fix)), str(tmp_path / 'back'), str(tmp_path / 'subs'),
+            '--couplings', str(tmp_path / 'J.npy'), '--alpha', 'ABCD',
+            '--outdir', str(tmp_path / ('out%d' % iterback))]
+    if iterback:
+        args.append('--iterbackgrounds')
+    got = Mi3.main(args)
+    Eb = po.energies(back, J)
+    if not iterback:
+        want = np.empty(len(subs))
+        for n, s in enumerate(subs):
+            h = back.copy()
+            h[:, fix] = s[fix]
+            want[n] = logsumexp(Eb - po.energies(h, J))
+    else:
+        want = np.full(len(subs), -np.inf)
+        other = [i for i in range(L) if i not in fix]
+        for n, b in enumerate(back):
+            h = subs.copy()
+            h[:, other] = b[other]
+            want = np.logaddexp(want, Eb[n] - po.energies(h, J))
+    np.testing.assert_allclose(got, want, atol=2e-4, rtol=1e-5)
+    assert (tmp_path / ('out%d' % iterback) / 'probs.npy').exists()

@@ -414,9 +414,10 @@ def test_SCADJ_piecewise_by_hand():
     f = np.full((1, 4), 0.25)
     for x, R in [(0.01, 0.1), (0.04, 0.4), (1.0, 0.25), (-1.2, 0.15), (2.0, 0.0)]:
         J = np.array([[x, -x, -x, x]])
-        got = po.reg_SCADJ(f, 0.1, 0.0, 0.5, 3.0, J, np.zeros_like(J))
+        got, amb = po.reg_SCADJ(f, 0.1, 0.0, 0.5, 3.0, J, np.zeros_like(J))
         exp = -R * 0.1 * np.sign(J) / 0.25
         np.testing.assert_allclose(got, exp, rtol=1e-12, atol=1e-15)
+        assert amb.all() == (abs(x) < 0.0505)
 
 
 def test_ddE_gauge_invariance_and_q2_closed_form():
