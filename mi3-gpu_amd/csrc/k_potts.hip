@@ -206,7 +206,10 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
     extern __shared__ f4v lds4[];
     char* ldsb = reinterpret_cast<char*>(lds4);
 
-    constexpr uint32_t chf = PF * NT * 4;  // padded chunk (floats)
+    // the padded chunk is PF float4 per thread of the first DMA_T threads
+    // (the chunk layout's unit); waves beyond them take no part in staging
+    constexpr int DMA_T = NT < 512 ? NT : 512;
+    constexpr uint32_t chf = PF * DMA_T * 4;  // padded chunk (floats)
     constexpr uint32_t chb = chf * 4;      // padded chunk (bytes)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -241,7 +244,11 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
     uint32_t Pe = 0, Po = 0;      // this lane's walker draws for steps 2h, 2h+1
 
     if (nsteps == 0) return;
-    chunk_dma<PF, NT>(rJ, voff, pos_list[0] * nchunk * chb, ldsb, wave);
+    // site of the current step; the next one is read one step ahead (with
+    // the prefetch of its first chunk), so no step waits on a scalar load
+    uint32_t pos = pos_list[0];
+    uint32_t pos_next = nsteps > 1 ? pos_list[1] : 0u;
+    if (wave < DMA_T / 64) chunk_dma<PF, DMA_T>(rJ, voff, pos * nchunk * chb, ldsb, wave);
     wait_vmem();
     __syncthreads();
     uint32_t buf = 0;
@@ -249,7 +256,7 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
     const uint32_t j4 = j * 4;
 
     for (uint32_t t = 0; t < nsteps; t++) {
-        const uint32_t pos = pos_list[t];
+        const uint32_t pos_after = (t + 2 < nsteps) ? pos_list[t + 2] : 0u;
         const uint32_t kp = pos >> 5, owner = pos & 31;
         const uint32_t kwp = kp >> 2, shp = (kp & 3) * 8;
         if ((t & 1) == 0) {
@@ -263,14 +270,25 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
         const uint32_t Pcur = (t & 1) ? Po : Pe;
         const int bmine = (int)(Pcur & 0xffu);
         const int own_addr = (int)((owner + (lane & 32)) * 4);
+        // residues a (from the owner lane of pos) and proposals b of every
+        // slot, fetched in groups ahead of their use
+        constexpr int GB = (S < 8) ? S : 8;
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            uint32_t av = (select_word_opaque<KW>(st[s], kwp) >> shp) & 0xffu;
-            uint32_t a = (uint32_t)__builtin_amdgcn_ds_bpermute(own_addr, (int)av);
-            uint32_t b = (uint32_t)half_bcast(bmine, s << SH);
-            baseA[s] = __umul24(a, qch4) + j4;
-            baseB[s] = __umul24(b, qch4) + j4;
-            acc[s] = 0.f;
+        for (int s0 = 0; s0 < S; s0 += GB) {
+            uint32_t a[GB], b[GB];
+#pragma unroll
+            for (int g = 0; g < GB; g++) {
+                const int s = s0 + g;
+                uint32_t av = (select_word_opaque<KW>(st[s], kwp) >> shp) & 0xffu;
+                a[g] = (uint32_t)__builtin_amdgcn_ds_bpermute(own_addr, (int)av);
+                b[g] = (uint32_t)half_bcast(bmine, s << SH);
+            }
+#pragma unroll
+            for (int g = 0; g < GB; g++) {
+                baseA[s0 + g] = __umul24(a[g], qch4) + j4;
+                baseB[s0 + g] = __umul24(b[g], qch4) + j4;
+                acc[s0 + g] = 0.f;
+            }
         }
         // stream the couplings of site pos, chunk by chunk
 #pragma unroll
@@ -279,10 +297,11 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
                 const bool last_chunk = (uint32_t)k + 1 == nchunk;
                 const bool has_next = !last_chunk || (t + 1 < nsteps);
                 if (has_next) {
-                    uint32_t npos = last_chunk ? pos_list[t + 1] : pos;
+                    uint32_t npos = last_chunk ? pos_next : pos;
                     uint32_t nc = last_chunk ? 0u : (uint32_t)k + 1;
-                    chunk_dma<PF, NT>(rJ, voff, (npos * nchunk + nc) * chb, ldsb + (buf ^ 1) * chb,
-                                      wave);
+                    if (wave < DMA_T / 64)
+                        chunk_dma<PF, DMA_T>(rJ, voff, (npos * nchunk + nc) * chb,
+                                             ldsb + (buf ^ 1) * chb, wave);
                 }
                 const char* cb = ldsb + buf * chb;
                 // gathers issued in groups of G slots ahead of their use, so
@@ -329,6 +348,8 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
                 }
             }
         }
+        pos = pos_next;
+        pos_next = pos_after;
     }
 
     // write back

@@ -20,16 +20,16 @@ hipError_t set_lds(F kernel, size_t bytes) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <int KW, int PF, int S>
+template <int KW, int PF, int S, int NW = NWAVES>
 hipError_t sweep_kws(const SweepArgs& a, hipStream_t st) {
-    constexpr uint32_t W = 2 * S * NWAVES;
-    auto kern = k_sweep<KW, S, NWAVES, PF>;
+    constexpr uint32_t W = 2 * S * NW;
+    auto kern = k_sweep<KW, S, NW, PF>;
     size_t lds = 2ull * chunk_floats(a.q) * sizeof(float);
     hipError_t e = set_lds(kern, lds);
     if (e != hipSuccess) return e;
     if (a.nwalk % W) return hipErrorInvalidValue;
     dim3 grid(a.nwalk / W);
-    hipLaunchKernelGGL(kern, grid, dim3(NT), lds, st, a.Ju, a.seq, a.pos, a.ld, a.L, a.q,
+    hipLaunchKernelGGL(kern, grid, dim3(64 * NW), lds, st, a.Ju, a.seq, a.pos, a.ld, a.L, a.q,
                        a.nchunk, a.nsteps, a.k0, a.k1, a.step0, a.walker_base, a.nbeta);
     return hipGetLastError();
 }
