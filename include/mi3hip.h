@@ -235,6 +235,47 @@ int mi3_reg_expX(mi3_ctx *ctx, float gamma, float pc, float lam);
 int mi3_reg_ddE(mi3_ctx *ctx, float gamma, float pc, float lam);
 int mi3_reg_SCADddE(mi3_ctx *ctx, float gamma, float pc, float lam, float r);
 
+/* ---- batched models -------------------------------------------------------
+ * Many independent Potts models of uneven L and q, each with its own
+ * couplings, walkers and random streams (model m is keyed by seed + m and
+ * runs exactly the chains a single-model context created with that seed
+ * would), swept together: one persistent kernel per class of models sharing
+ * the register layout (ceil(L/128) state words, the chunk class of q), the
+ * classes on parallel branches of one HIP graph per round.  No counterpart
+ * in the reference, which runs one model per process. */
+typedef struct mi3_batch mi3_batch;
+
+/* nmodels models, model m with L[m] sites, q[m] letters, nwalkers[m]
+ * walkers (a multiple of 512).  Couplings start at zero, walkers at
+ * residue 0. */
+int mi3_batch_create(int device, uint32_t nmodels, const uint32_t *L, const uint32_t *q,
+                     const uint32_t *nwalkers, uint64_t seed, mi3_batch **out);
+int mi3_batch_destroy(mi3_batch *b);
+const char *mi3_batch_last_error(mi3_batch *b);
+
+/* Packed (L(L-1)/2, q*q) float32 couplings of one model. */
+int mi3_batch_set_couplings(mi3_batch *b, uint32_t model, const float *J);
+
+/* Walkers of one model, site-major: L rows of nwalkers residues. */
+int mi3_batch_set_seqs(mi3_batch *b, uint32_t model, const uint8_t *seqs);
+int mi3_batch_get_seqs(mi3_batch *b, uint32_t model, uint8_t *out);
+
+/* nsweeps sweeps (nsweeps*L site updates per walker) of every model: one
+ * graph launch; the graph is captured on the first call for a given
+ * nsweeps and after any tile change. */
+int mi3_batch_run(mi3_batch *b, uint32_t nsweeps);
+int mi3_batch_synchronize(mi3_batch *b);
+
+/* Autotuner: for every class, time each allowed walkers-per-tile choice
+ * (128, 256, 512; 128 only for L > 256) on nsweeps sweeps of its models and
+ * keep the fastest.  The timed sweeps are real MCMC steps of the walkers. */
+int mi3_batch_tune(mi3_batch *b, uint32_t nsweeps);
+int mi3_batch_tile(mi3_batch *b, uint32_t model, uint32_t *wpb);
+int mi3_batch_set_tile(mi3_batch *b, uint32_t model, uint32_t wpb);
+
+/* Global step counter of a model's random streams. */
+int mi3_batch_step_count(mi3_batch *b, uint32_t model, uint64_t *steps);
+
 /* ---- profiling ------------------------------------------------------------ */
 
 /* Append "EVT name start_ns end_ns bytes" lines for every completed launch to

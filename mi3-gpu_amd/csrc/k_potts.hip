@@ -18,6 +18,7 @@
 //   Ju[i][c][a*q + b][jl] = J_{i,m}(a, b),  m = 32c + jl,
 // symmetric (both i<m and i>m filled), zero for m == i and m >= L.
 #include "common.h"
+#include "launch.h"
 
 namespace mi3 {
 
@@ -194,11 +195,15 @@ __device__ __forceinline__ uint32_t lane_select_u(uint64_t mask, uint32_t a, uin
 // broadcast), gathers J(a, s_m) and J(b, s_m) for its sites chunk by chunk,
 // and one transposed butterfly leaves each lane with the dE of its own slot;
 // it decides, and a ballot tells the owner lane of pos which walkers accepted.
+//
+// sweep_block runs the walkers of work-group tile `block` (W = 2*S*NWAVES
+// walkers starting at walker block*W); k_sweep is one tile per work-group,
+// k_sweep_batch a persistent loop over the tiles of many models.
 template <int KW, int S, int NWAVES, int PF>
-__global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_sweep(
+__device__ __forceinline__ void sweep_block(
     const float* __restrict__ Ju, uint8_t* __restrict__ seq, const uint32_t* __restrict__ pos_list,
     uint32_t ldseq, uint32_t L, uint32_t q, uint32_t nchunk, uint32_t nsteps, uint32_t k0,
-    uint32_t k1, uint64_t step0, uint32_t walker_base, float nbeta) {
+    uint32_t k1, uint64_t step0, uint32_t walker_base, float nbeta, uint32_t block) {
     constexpr int NT = 64 * NWAVES;
     constexpr int K = 4 * KW;
     constexpr int SH = 5 - ilog2(S);  // lane j reduces and decides slot j >> SH
@@ -216,7 +221,7 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
     const uint32_t j = lane & 31;
     const bool upper = lane >= 32;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const uint32_t wfirst = blockIdx.x * (2 * S * NWAVES) + (wave * 2 + (upper ? 1 : 0)) * S;
+    const uint32_t wfirst = block * (2 * S * NWAVES) + (wave * 2 + (upper ? 1 : 0)) * S;
     const uint32_t myslot = j >> SH;
     const __amdgpu_buffer_rsrc_t rJ = make_rsrc(Ju, L * nchunk * chb);
     const uint32_t voff = tid * 16;
@@ -246,8 +251,10 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
     if (nsteps == 0) return;
     // site of the current step; the next one is read one step ahead (with
     // the prefetch of its first chunk), so no step waits on a scalar load
-    uint32_t pos = pos_list[0];
-    uint32_t pos_next = nsteps > 1 ? pos_list[1] : 0u;
+    // (readfirstlane: the site is wave-uniform whatever the compiler can prove
+    // about the list's memory, so masks built from it stay in SGPRs)
+    uint32_t pos = __builtin_amdgcn_readfirstlane(pos_list[0]);
+    uint32_t pos_next = nsteps > 1 ? __builtin_amdgcn_readfirstlane(pos_list[1]) : 0u;
     if (wave < DMA_T / 64) chunk_dma<PF, DMA_T>(rJ, voff, pos * nchunk * chb, ldsb, wave);
     wait_vmem();
     __syncthreads();
@@ -256,7 +263,8 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
     const uint32_t j4 = j * 4;
 
     for (uint32_t t = 0; t < nsteps; t++) {
-        const uint32_t pos_after = (t + 2 < nsteps) ? pos_list[t + 2] : 0u;
+        const uint32_t pos_after =
+            (t + 2 < nsteps) ? __builtin_amdgcn_readfirstlane(pos_list[t + 2]) : 0u;
         const uint32_t kp = pos >> 5, owner = pos & 31;
         const uint32_t kwp = kp >> 2, shp = (kp & 3) * 8;
         if ((t & 1) == 0) {
@@ -363,6 +371,51 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
                 if (m < L) seq[(size_t)m * ldseq + wfirst + s] = (uint8_t)(st[s][kw] >> (8 * e));
             }
         }
+    }
+}
+
+
+template <int KW, int S, int NWAVES, int PF>
+__global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_sweep(
+    const float* __restrict__ Ju, uint8_t* __restrict__ seq, const uint32_t* __restrict__ pos_list,
+    uint32_t ldseq, uint32_t L, uint32_t q, uint32_t nchunk, uint32_t nsteps, uint32_t k0,
+    uint32_t k1, uint64_t step0, uint32_t walker_base, float nbeta) {
+    sweep_block<KW, S, NWAVES, PF>(Ju, seq, pos_list, ldseq, L, q, nchunk, nsteps, k0, k1, step0,
+                                   walker_base, nbeta, blockIdx.x);
+}
+
+// Persistent sweep over the tiles of one (KW, PF) class of a batch of
+// models: every work-group takes tiles from a shared counter until none is
+// left (the counter is cleared before each launch).
+template <int KW, int S, int NWAVES, int PF>
+__global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void
+k_sweep_batch(const BatchModel* __restrict__ models, const BatchTile* __restrict__ tiles,
+              uint32_t ntiles, uint32_t* __restrict__ counter) {
+    __shared__ uint32_t s_tile;
+    for (;;) {
+        __syncthreads();   // previous tile's LDS reads done; s_tile free
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile);  // uniform: scalar loads below
+        if (t >= ntiles) break;
+        const BatchTile tl = tiles[t];
+        const BatchModel& m = models[tl.model];
+        // descriptor fields are uniform; say so (the kernel's own stores to
+        // walker memory keep the compiler from proving it)
+        auto u32 = [](uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane(v); };
+        // (readfirstlane returns int: widen through uint32_t, or a low word
+        // with bit 31 set would sign-extend over the high word)
+        auto ptr = [](const void* p) {
+            const uint64_t v = (uint64_t)p;
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+            return (uint64_t)lo | ((uint64_t)hi << 32);
+        };
+        const uint64_t step0 = ptr((const void*)*m.step);
+        sweep_block<KW, S, NWAVES, PF>(
+            (const float*)ptr(m.Ju), (uint8_t*)ptr(m.seq), (const uint32_t*)ptr(m.pos), u32(m.ld),
+            u32(m.L), u32(m.q), u32(m.nchunk), u32(m.nsteps), u32(m.k0), u32(m.k1), step0,
+            u32(m.walker_base), __uint_as_float(u32(__float_as_uint(m.nbeta))), u32(tl.block));
     }
 }
 

@@ -48,8 +48,39 @@ inline uint32_t pf_class(uint32_t q) {
 // Floats per (padded) chunk.
 inline uint32_t chunk_floats(uint32_t q) { return pf_class(q) * POTTS_THREADS * 4; }
 
+// One model of a batch, as the persistent sweep reads it (device memory).
+struct BatchModel {
+    const float* Ju;        // chunked couplings
+    uint8_t* seq;           // site-major walkers, L rows of ld bytes
+    const uint32_t* pos;    // site list of this launch
+    const uint64_t* step;   // global step counter (even)
+    uint32_t ld, L, q, nchunk, nsteps, k0, k1, walker_base;
+    float nbeta;
+    uint32_t pad_;
+};
+
+// A work-group tile of a batch: walkers [block*W, (block+1)*W) of `model`.
+struct BatchTile {
+    uint32_t model, block;
+};
+
 // Returns a hipError_t; grid/LDS sizing done inside.
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 hipError_t launch_energies(const EnergyArgs& a, hipStream_t st);
+
+// State words per lane (4 residues each) for L sites: ceil(L / 128).
+inline uint32_t state_words(uint32_t L) { return (L + 127) / 128; }
+
+// Persistent sweep over `ntiles` tiles of models that share state_words(L)
+// == kw and pf_class(q) == pf, with `wpb` walkers per tile (128, 256 or 512;
+// 128 only for kw >= 3), on `grid` work-groups.  `counter` (one u32) must
+// be zero at launch.
+hipError_t launch_sweep_batch(uint32_t kw, uint32_t pf, uint32_t wpb, const BatchModel* models,
+                              const BatchTile* tiles, uint32_t ntiles, uint32_t* counter,
+                              uint32_t grid, hipStream_t st);
+
+// Chunked symmetric coupling layout from packed couplings (k_unpack_chunked).
+hipError_t launch_unpack_chunked(const float* J, float* out, uint32_t L, uint32_t q,
+                                 uint32_t nchunk, hipStream_t st);
 
 }  // namespace mi3

@@ -86,6 +86,47 @@ hipError_t launch_sweep(const SweepArgs& a, hipStream_t st) {
     return dispatch<SweepDisp>(a, st, a.nchunk, a.q);
 }
 
+namespace {
+template <int KW, int PF, int S>
+hipError_t batch_kws(const BatchModel* models, const BatchTile* tiles, uint32_t ntiles,
+                     uint32_t* counter, uint32_t grid, hipStream_t st) {
+    auto kern = k_sweep_batch<KW, S, NWAVES, PF>;
+    size_t lds = 2ull * PF * POTTS_THREADS * 4 * sizeof(float);  // two padded chunks
+    hipError_t e = set_lds(kern, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, models, tiles, ntiles, counter);
+    return hipGetLastError();
+}
+
+template <int KW, int PF>
+hipError_t batch_kw(uint32_t wpb, const BatchModel* models, const BatchTile* tiles,
+                    uint32_t ntiles, uint32_t* counter, uint32_t grid, hipStream_t st) {
+    if constexpr (KW <= 2) {
+        if (wpb == 128) return batch_kws<KW, PF, 8>(models, tiles, ntiles, counter, grid, st);
+        if (wpb == 256) return batch_kws<KW, PF, 16>(models, tiles, ntiles, counter, grid, st);
+        if (wpb == 512) return batch_kws<KW, PF, 32>(models, tiles, ntiles, counter, grid, st);
+        return hipErrorInvalidValue;
+    } else {
+        if (wpb != 128) return hipErrorInvalidValue;
+        return batch_kws<KW, PF, 8>(models, tiles, ntiles, counter, grid, st);
+    }
+}
+}  // namespace
+
+hipError_t launch_sweep_batch(uint32_t kw, uint32_t pf, uint32_t wpb, const BatchModel* models,
+                              const BatchTile* tiles, uint32_t ntiles, uint32_t* counter,
+                              uint32_t grid, hipStream_t st) {
+    if (grid == 0 || ntiles == 0) return hipSuccess;
+#define MI3_BCASE(KW_, PF_) \
+    if (kw == KW_ && pf == PF_) return batch_kw<KW_, PF_>(wpb, models, tiles, ntiles, counter, grid, st);
+    MI3_BCASE(1, 1) MI3_BCASE(1, 4) MI3_BCASE(1, 7) MI3_BCASE(1, 10)
+    MI3_BCASE(2, 1) MI3_BCASE(2, 4) MI3_BCASE(2, 7) MI3_BCASE(2, 10)
+    MI3_BCASE(3, 1) MI3_BCASE(3, 4) MI3_BCASE(3, 7) MI3_BCASE(3, 10)
+    MI3_BCASE(4, 1) MI3_BCASE(4, 4) MI3_BCASE(4, 7) MI3_BCASE(4, 10)
+#undef MI3_BCASE
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_energies(const EnergyArgs& a, hipStream_t st) {
     if (a.q > MAX_Q || a.L > MAX_L || a.L < 2) return hipErrorInvalidValue;
     if (a.nseq == 0) return hipSuccess;

@@ -17,6 +17,17 @@
 
 using namespace mi3;
 
+namespace mi3 {
+hipError_t launch_unpack_chunked(const float* J, float* out, uint32_t L, uint32_t q,
+                                 uint32_t nchunk, hipStream_t st) {
+    const uint32_t chf = chunk_floats(q);
+    const uint64_t total = (uint64_t)L * nchunk * chf;
+    hipLaunchKernelGGL(k_unpack_chunked, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, st,
+                       J, out, L, q, nchunk, chf, total);
+    return hipGetLastError();
+}
+}  // namespace mi3
+
 namespace {
 
 struct Buf {

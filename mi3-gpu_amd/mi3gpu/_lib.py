@@ -60,6 +60,19 @@ SIGNATURES = {
     'mi3_reg_l2z': (_i, [_P, _f, _f, _f]),
     'mi3_update_J_reg': (_i, [_P, _f, _f, _i, _f]),
     'mi3_copy_seqs': (_i, [_P, _S, _u32, _S, _u32, _u32]),
+    'mi3_batch_create': (_i, [_i, _u32, _c.POINTER(_u32), _c.POINTER(_u32), _c.POINTER(_u32),
+                              _u64, _c.POINTER(_P)]),
+    'mi3_batch_destroy': (_i, [_P]),
+    'mi3_batch_last_error': (_S, [_P]),
+    'mi3_batch_set_couplings': (_i, [_P, _u32, _P]),
+    'mi3_batch_set_seqs': (_i, [_P, _u32, _P]),
+    'mi3_batch_get_seqs': (_i, [_P, _u32, _P]),
+    'mi3_batch_run': (_i, [_P, _u32]),
+    'mi3_batch_synchronize': (_i, [_P]),
+    'mi3_batch_tune': (_i, [_P, _u32]),
+    'mi3_batch_tile': (_i, [_P, _u32, _c.POINTER(_u32)]),
+    'mi3_batch_set_tile': (_i, [_P, _u32, _u32]),
+    'mi3_batch_step_count': (_i, [_P, _u32, _c.POINTER(_u64)]),
     'mi3_copy_subseq': (_i, [_P, _S, _u32, _S, _S]),
     'mi3_reg_SCADJ': (_i, [_P, _f, _f, _f, _f]),
     'mi3_reg_X': (_i, [_P, _f, _f, _f]),
