@@ -159,6 +159,48 @@ def measure_rounds(target, L, q, N, rounds, seed):
             "note": "rounds from the independent-site model; SSR column per round"}
 
 
+def measure_batch(rounds, sweeps=64, nmodels=64, seed=4):
+    """Batched uneven models: nmodels Potts models with L ~ U{50..400},
+    q in {4, 21} (probability 1/2 each), h ~ N(0,1), J ~ N(0,0.05) in the
+    zero-sum gauge folded to fieldless float32, 512*L walkers each, `sweeps`
+    sweeps per round in one graph launch, tiles chosen by the autotuner."""
+    from mi3gpu.batch import PottsBatch
+    rng = np.random.default_rng(seed)
+    Ls = rng.integers(50, 401, size=nmodels)
+    qs = np.where(rng.random(nmodels) < 0.5, 4, 21)
+    Ns = 512 * Ls
+    b = PottsBatch(Ls, qs, Ns, seed=seed)
+    for m, (L, q) in enumerate(zip(Ls, qs)):
+        b.set_couplings(m, production_model(int(L), int(q), seed=1000 + m))
+        b.set_seqs(m, rng.integers(0, q, size=(int(512 * L), int(L))).astype('u1'))
+        if m % 16 == 15:
+            print(f"batch: {m + 1} models set up", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b.tune(1)
+    b.wait()
+    t_tune = time.perf_counter() - t0
+    print(f"batch: tuned in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    b.run(sweeps)                 # capture + one warm round
+    b.wait()
+    print("batch: warm round done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        b.run(sweeps)
+    b.wait()
+    dt = (time.perf_counter() - t0) / rounds
+    su = float(sum(int(n) * int(L) * sweeps for n, L in zip(Ns, Ls)))
+    tiles = {}
+    for m, (L, q) in enumerate(zip(Ls, qs)):
+        key = f"L{'<=128' if L <= 128 else ('<=256' if L <= 256 else '>256')} q={q}"
+        tiles[key] = b.tile(m)
+    b.close()
+    return {"value": su / dt, "unit": "site-updates/s", "s_per_round": dt, "rounds": rounds,
+            "models": nmodels, "sweeps_per_round": sweeps, "walkers": int(Ns.sum()),
+            "site_updates_per_round": su, "tiles": tiles, "tune_s": t_tune,
+            "workload": "64 models, L~U{50..400}, q in {4,21}, 512*L walkers each, "
+                        "64 sweeps per round in one HIP graph"}
+
+
 def copy_calibration(mib):
     """Plain device-to-device copies of a known size: a profiler run that
     also covers them turns FETCH_SIZE / WRITE_SIZE counts into bytes."""
@@ -192,6 +234,8 @@ def main():
                     help="sweeps to sample the 2^20 target sequences")
     ap.add_argument('--rounds', type=int, default=2,
                     help="Mi3 inference rounds to time (0 = skip)")
+    ap.add_argument('--batch-rounds', type=int, default=0,
+                    help="also time the batched uneven-models workload (rounds)")
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help="also run 3 device-to-device copies of MIB MiB (the reference "
                          "kernel that scales HBM byte counters)")
@@ -265,6 +309,8 @@ def main():
         del tseqs
         if a.rounds > 0:
             out["round"] = measure_rounds(target, L, q, N, a.rounds, 6)
+    if a.batch_rounds > 0:
+        out["batch"] = measure_batch(a.batch_rounds)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(J, L, q, seed=3)
     print(json.dumps(out), flush=True)

@@ -70,7 +70,10 @@ struct SweepDisp {
             if (w <= 256) return sweep_kws<KW, PF, 16>(a, st);
             return sweep_kws<KW, PF, 32>(a, st);
         } else {
-            return sweep_kws<KW, PF, 8>(a, st);
+            // more state words per walker: at most 16 slots fit in registers
+            uint32_t w = a.wpb ? a.wpb : SWEEP_DEFAULT_WPB;
+            if (w <= 128) return sweep_kws<KW, PF, 8>(a, st);
+            return sweep_kws<KW, PF, 16>(a, st);
         }
     }
 };
@@ -107,8 +110,9 @@ hipError_t batch_kw(uint32_t wpb, const BatchModel* models, const BatchTile* til
         if (wpb == 512) return batch_kws<KW, PF, 32>(models, tiles, ntiles, counter, grid, st);
         return hipErrorInvalidValue;
     } else {
-        if (wpb != 128) return hipErrorInvalidValue;
-        return batch_kws<KW, PF, 8>(models, tiles, ntiles, counter, grid, st);
+        if (wpb == 128) return batch_kws<KW, PF, 8>(models, tiles, ntiles, counter, grid, st);
+        if (wpb == 256) return batch_kws<KW, PF, 16>(models, tiles, ntiles, counter, grid, st);
+        return hipErrorInvalidValue;
     }
 }
 }  // namespace

@@ -64,8 +64,7 @@ def test_batch_tiles_give_identical_chains(gpu_ok, wpb):
     b1, Js, S0 = _setup()
     b2, _, _ = _setup()
     for m, (L, q, N) in enumerate(MODELS):
-        if L <= 256:
-            b2.set_tile(m, wpb)
+        b2.set_tile(m, wpb)
     b1.run(2)
     b2.run(2)
     for m in range(len(MODELS)):
@@ -79,12 +78,12 @@ def test_batch_tune_and_errors(gpu_ok):
     b, Js, S0 = _setup()
     b.tune(1)
     for m, (L, q, N) in enumerate(MODELS):
-        assert b.tile(m) in ((128,) if L > 256 else (128, 256, 512))
+        assert b.tile(m) in ((128, 256) if L > 256 else (128, 256, 512))
         assert b.step_count(m) > 0
     b.run(1)
     b.wait()
     with pytest.raises(ValueError):
-        b.set_tile(4, 512)            # L = 260 needs 128
+        b.set_tile(4, 512)            # L = 260 allows at most 256
     with pytest.raises(ValueError):
         b.set_seqs(0, np.full((512, 20), 9, 'u1'))
     with pytest.raises(ValueError):
