@@ -53,17 +53,17 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {(232, 21, 2 ** 18, 8, 512): 13.42e9}
 
 
-def production_model(L=232, q=21, seed=1):
-    """h ~ N(0,1), J ~ N(0,0.05), zero-sum gauge, folded to fieldless float32."""
+def production_model(L=232, q=21, seed=1, hs=1.0, js=0.05):
+    """h ~ N(0,hs), J ~ N(0,js), zero-sum gauge, folded to fieldless float32."""
     from mi3gpu.utils.changeGauge import zeroJGauge, fieldlessGaugeEven
     rng = np.random.default_rng(seed)
-    h = rng.normal(0, 1.0, size=(L, q))
-    J = rng.normal(0, 0.05, size=(L * (L - 1) // 2, q * q))
+    h = rng.normal(0, hs, size=(L, q))
+    J = rng.normal(0, js, size=(L * (L - 1) // 2, q * q))
     h, J = zeroJGauge(h, J)
     return fieldlessGaugeEven(h, J)[1].astype('<f4')
 
 
-def cpu_baseline(J, L, q, seed, target_s=12.0):
+def cpu_baseline(J, L, q, seed, target_s=20.0):
     """Time the CPU port of the sampler on this host's cores."""
     from oracle import cboracle
     nthr = int(min(16, os.cpu_count() or 1))
@@ -201,6 +201,101 @@ def measure_batch(rounds, sweeps=64, nmodels=64, seed=4):
                         "64 sweeps per round in one HIP graph"}
 
 
+def measure_kernels(n=2 ** 20, L=232, q=21, reps=3):
+    """Bimarginal counts (int8 one-hot Gram product on the matrix cores) and
+    batch energies standalone on n uniform i.i.d. sequences: average kernel
+    time (HIP events) and rates.  Useful int8 MACs of the counts:
+    n*q^2*L(L-1)/2; energies: n*L(L-1)/2 coupling gathers."""
+    from mi3gpu.mcmcGPU import MCMCGPU
+    J = production_model(L, q, seed=1)
+    seqs = np.random.default_rng(3).integers(0, q, size=(n, L)).astype('u1')
+    with tempfile.TemporaryDirectory() as d:
+        g = MCMCGPU((0, 0, None, None), L, q, n, 512, Path(d), 1024, 2, profile=True)
+        g.setBuf('J', J)
+        g.setBuf('seq main', seqs)
+        g.calcBicounts('main')
+        g.calcEnergies('main')
+        g.wait()
+        g.profile_reset()
+        for _ in range(reps):
+            g.calcBicounts('main')
+            g.calcEnergies('main')
+        g.wait()
+        bc_ms, _ = g.kernel_ms('calcBicounts')
+        en_ms, _ = g.kernel_ms('getEnergies')
+        bc = g.getBuf('bicount').read()
+        del g
+    nP = L * (L - 1) // 2
+    macs = float(n) * q * q * nP
+    ok_rows = bool(np.all(bc.sum(axis=1) == n))
+    return {"sequences": n, "L": L, "q": q,
+            "bicounts_ms": bc_ms, "bicounts_int8_TOPS": 2 * macs / (bc_ms * 1e-3) / 1e12,
+            "bicounts_frac_of_5.0_POPS": 2 * macs / (bc_ms * 1e-3) / 5.0e15,
+            "bicount_rows_sum_to_n": ok_rows,
+            "energies_ms": en_ms, "energy_gathers_per_s": float(n) * nP / (en_ms * 1e-3)}
+
+
+def measure_config1(seed=0):
+    """The reference's CPU-runnable round: L=99, q=4, h ~ N(0,0.5),
+    J ~ N(0,0.1) (zero-sum gauge, fieldless float32); target bimarginals of
+    10,000 sequences after 2^14 sweeps (+ Jeffreys pseudocount); one round of
+    16,384 walkers x 64 sweeps + bicounts + quasi-Newton phase (reference
+    defaults).  GPU: `Mi3.py infer`, one round, wall time.  CPU: the
+    reference path's parts timed on this host -- the sampler (C port,
+    OpenMP), numpy bicounts (one core) and numpy quasi-Newton iterations (one
+    core; two timed, scaled by the GPU round's iteration count)."""
+    import contextlib
+    import io
+    import mi3gpu.Mi3 as Mi3
+    from mi3gpu.mcmcGPU import MCMCGPU
+    from mi3gpu.utils.pseudocount import mutation_pc
+    from oracle import cboracle, potts_oracle as po
+    L, q, N = 99, 4, 16384
+    J = production_model(L, q, seed=seed, hs=0.5, js=0.1)
+    with tempfile.TemporaryDirectory() as d:
+        nt = 10240
+        g = MCMCGPU((0, 0, None, None), L, q, nt, 512, Path(d), 1024, seed)
+        g.initMCMC(64 * L, 0, 2 ** 63)
+        g.setBuf('J', J)
+        g.setBuf('seq main', np.random.default_rng(seed).integers(0, q, (nt, L)).astype('u1'))
+        for _ in range(2 ** 14 // 64):
+            g.runMCMC()
+        tseq = g.getBuf('seq main').read()[:10000]
+        del g
+        f = cboracle.bicounts(tseq, q).astype(np.float64) / 10000
+        target = mutation_pc(f, 10000).astype('<f4')
+        np.save(Path(d) / 'target.npy', target)
+        args = ['infer', '--init_model', 'independent', '--bimarg', str(Path(d) / 'target.npy'),
+                '--alpha', 'ABCD', '--nwalkers', str(N), '--nsteps', str(8 * L),
+                '--equiltime', '8', '--mcsteps', '1', '--rngseed', '7',
+                '--outdir', str(Path(d) / 'infer')]
+        with contextlib.redirect_stdout(io.StringIO()):
+            p = Mi3.main(args)
+        gpu = p.round_stats[0]
+        # CPU parts on the same sizes
+        nthr = int(min(16, os.cpu_count() or 1))
+        s0 = np.random.default_rng(1).integers(0, q, (N, L)).astype('u1')
+        t0 = time.perf_counter()
+        s1 = cboracle.metropolis_f32(s0, J, 7, 0, 64 * L, nthreads=nthr)
+        t_samp = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        bc = po.bicounts(s1, q)
+        t_bc = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        it, *_ = po.iter_newton(s1, J, target, 4e-4, 1e-3, 2, 0.0)
+        t_it = (time.perf_counter() - t0) / (it + 1)
+    n_newton = gpu['newton_steps']
+    cpu_round = t_samp + t_bc + max(1, n_newton) * t_it
+    return {"workload": "L=99 q=4 round: 16384 walkers x 64 sweeps, bicounts, quasi-Newton",
+            "gpu_round_s": gpu['wall_s'], "gpu_newton_steps": n_newton,
+            "gpu_mcmc_s": gpu['mcmc_s'], "gpu_newton_s": gpu['newton_s'],
+            "cpu_round_s_est": cpu_round, "cpu_sampler_s": t_samp, "cpu_sampler_threads": nthr,
+            "cpu_bicounts_s": t_bc, "cpu_newton_iter_s": t_it,
+            "speedup_round": cpu_round / gpu['wall_s'],
+            "note": "CPU round = sampler + numpy bicounts + GPU's Newton iteration count x "
+                    "measured numpy iteration time (2 iterations timed)"}
+
+
 def copy_calibration(mib):
     """Plain device-to-device copies of a known size: a profiler run that
     also covers them turns FETCH_SIZE / WRITE_SIZE counts into bytes."""
@@ -234,6 +329,10 @@ def main():
                     help="sweeps to sample the 2^20 target sequences")
     ap.add_argument('--rounds', type=int, default=2,
                     help="Mi3 inference rounds to time (0 = skip)")
+    ap.add_argument('--config1', action='store_true',
+                    help="also time the L=99 q=4 round on the GPU and its CPU reference path")
+    ap.add_argument('--kernels', action='store_true',
+                    help="also time bicounts and energies standalone on 2^20 sequences")
     ap.add_argument('--batch-rounds', type=int, default=0,
                     help="also time the batched uneven-models workload (rounds)")
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -309,6 +408,10 @@ def main():
         del tseqs
         if a.rounds > 0:
             out["round"] = measure_rounds(target, L, q, N, a.rounds, 6)
+    if a.kernels:
+        out["kernels"] = measure_kernels()
+    if a.config1:
+        out["config1_round"] = measure_config1()
     if a.batch_rounds > 0:
         out["batch"] = measure_batch(a.batch_rounds)
     if not a.no_cpu:
