@@ -2,11 +2,11 @@
 //
 // With X[n][r] = [s_{n, r/q} == r % q] (N x L*q, one-hot, never stored),
 // count_{ij}(a, b) = (X^T X)[i*q + a][j*q + b] for i < j.  Each wave owns a
-// 64 x 64 block of the flattened L*q x L*q Gram matrix (2 x 2 tiles of
+// 128 x 128 block of the flattened L*q x L*q Gram matrix (4 x 4 tiles of
 // v_mfma_i32_32x32x32_i8), builds its int8 operands on the fly from the
 // site-major residue bytes (16 sequences per lane per k-step), contracts
-// over all sequences with int32 accumulation, and scatters the i < j entries
-// into the packed (L(L-1)/2, q*q) uint32 table.  Integer accumulation makes
+// over its range of sequences with int32 accumulation, and scatters the
+// i < j entries into the packed (L(L-1)/2, q*q) uint32 table.  Integer accumulation makes
 // the result exact whatever the summation order.
 // Replaces countBivariate (mcmc.cl:464-516); matches np.bincount per pair
 // (getMarginals.py:80-86).
@@ -51,13 +51,20 @@ __device__ __forceinline__ v4i onehot16(const uint8_t* __restrict__ seq, uint32_
     return r;
 }
 
-// grid: one wave per (row block rb, col block cb) with rb <= cb, blocks of 64.
-__global__ __launch_bounds__(64) void k_bicounts_mfma(const uint8_t* __restrict__ seq,
-                                                      uint32_t ldseq, uint32_t nseq, uint32_t L,
-                                                      uint32_t q, uint32_t nblk,
-                                                      uint32_t* __restrict__ counts) {
-    // decode the triangular block index
-    uint32_t t = blockIdx.x;
+// grid: ksplit waves per (row block rb, col block cb), rb <= cb, blocks of
+// 128 (256 int32 accumulators per lane, held in AGPRs), so every one-hot
+// operand built from a 16-byte load feeds 4 matrix instructions.  The
+// sequence range is split over the ksplit waves of a block, whose partial
+// counts are added with integer atomics (exact in any order; the table must
+// be zeroed first).  The next k-step's residue bytes are loaded before the
+// current step's products.
+__global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restrict__ seq,
+                                                         uint32_t ldseq, uint32_t nseq,
+                                                         uint32_t L, uint32_t q, uint32_t nblk,
+                                                         uint32_t ksplit,
+                                                         uint32_t* __restrict__ counts) {
+    uint32_t t = blockIdx.x / ksplit;
+    const uint32_t part = blockIdx.x - t * ksplit;
     uint32_t rb = 0;
     while (t >= nblk - rb) {
         t -= nblk - rb;
@@ -67,58 +74,108 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma(const uint8_t* __restrict_
     const uint32_t l = threadIdx.x;
     const uint32_t lr = l & 31, g = l >> 5;
     const uint32_t Lq = L * q;
+    // sequence range of this part, in steps of 32
+    const uint32_t nsteps = (nseq + 31) / 32;
+    const uint32_t s0 = (uint32_t)(((uint64_t)nsteps * part) / ksplit);
+    const uint32_t s1 = (uint32_t)(((uint64_t)nsteps * (part + 1)) / ksplit);
 
-    // operand rows of this lane: two row tiles (A) and two column tiles (B)
-    uint32_t siteA[2], a4A[2], siteB[2], a4B[2];
-    bool vA[2], vB[2];
+    uint32_t offA[4], a4A[4], offB[4], a4B[4];
+    bool vA[4], vB[4];
 #pragma unroll
-    for (int x = 0; x < 2; x++) {
-        uint32_t r = rb * 64 + x * 32 + lr;
+    for (int x = 0; x < 4; x++) {
+        const uint32_t r = rb * 128 + x * 32 + lr;
         vA[x] = r < Lq;
-        siteA[x] = vA[x] ? r / q : 0;
-        a4A[x] = (r - siteA[x] * q) * 0x01010101u;
-        uint32_t c = cb * 64 + x * 32 + lr;
+        const uint32_t si = vA[x] ? r / q : 0;
+        a4A[x] = (r - si * q) * 0x01010101u;
+        offA[x] = si * ldseq + 16 * g;
+        const uint32_t c = cb * 128 + x * 32 + lr;
         vB[x] = c < Lq;
-        siteB[x] = vB[x] ? c / q : 0;
-        a4B[x] = (c - siteB[x] * q) * 0x01010101u;
+        const uint32_t sj = vB[x] ? c / q : 0;
+        a4B[x] = (c - sj * q) * 0x01010101u;
+        offB[x] = sj * ldseq + 16 * g;
     }
-    v16i acc[2][2];
+    v16i acc[4][4];
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int x = 0; x < 4; x++)
 #pragma unroll
-        for (int y = 0; y < 2; y++)
+        for (int y = 0; y < 4; y++)
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[x][y][e] = 0;
 
-    for (uint32_t n0 = 0; n0 < nseq; n0 += 32) {
-        const uint32_t n = n0 + 16 * g;
-        v4i A0 = onehot16(seq, ldseq, siteA[0], a4A[0], vA[0], n, nseq);
-        v4i A1 = onehot16(seq, ldseq, siteA[1], a4A[1], vA[1], n, nseq);
-        v4i B0 = onehot16(seq, ldseq, siteB[0], a4B[0], vB[0], n, nseq);
-        v4i B1 = onehot16(seq, ldseq, siteB[1], a4B[1], vB[1], n, nseq);
-        acc[0][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A0, B0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A0, B1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A1, B0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A1, B1, acc[1][1], 0, 0, 0);
+    auto load = [&](uint32_t step, uint4 (&ra)[4], uint4 (&rbv)[4]) {
+        const uint32_t n = step * 32;
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            ra[x] = vA[x] ? *reinterpret_cast<const uint4*>(seq + offA[x] + n) : make_uint4(0, 0, 0, 0);
+            rbv[x] = vB[x] ? *reinterpret_cast<const uint4*>(seq + offB[x] + n) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto onehot = [](const uint4& v, uint32_t a4, bool valid) {
+        v4i r = {0, 0, 0, 0};
+        if (valid) {
+            r[0] = (int)eq_bytes(v.x, a4);
+            r[1] = (int)eq_bytes(v.y, a4);
+            r[2] = (int)eq_bytes(v.z, a4);
+            r[3] = (int)eq_bytes(v.w, a4);
+        }
+        return r;
+    };
+    // full steps (all 32 sequences < nseq); the ragged last step, if any,
+    // goes through the byte-wise path
+    const uint32_t full_end = nseq / 32;
+    const uint32_t e1 = s1 < full_end ? s1 : full_end;
+    uint4 ra[4], rbv[4];
+    if (s0 < e1) load(s0, ra, rbv);
+    for (uint32_t st = s0; st < e1; st++) {
+        v4i A[4], B[4];
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            A[x] = onehot(ra[x], a4A[x], vA[x]);
+            B[x] = onehot(rbv[x], a4B[x], vB[x]);
+        }
+        if (st + 1 < e1) load(st + 1, ra, rbv);
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++)
+                acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[x], B[y], acc[x][y], 0, 0, 0);
+    }
+    if (s1 > full_end && full_end >= s0) {
+        // ragged tail: the last, partial step of this part
+        const uint32_t n = full_end * 32 + 16 * g;
+        v4i A[4], B[4];
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            A[x] = onehot16(seq, ldseq, offA[x] / ldseq, a4A[x], vA[x], n, nseq);
+            B[x] = onehot16(seq, ldseq, offB[x] / ldseq, a4B[x], vB[x], n, nseq);
+        }
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++)
+                acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[x], B[y], acc[x][y], 0, 0, 0);
     }
 
-    // D layout of a 32x32 MFMA tile: value e of lane l is row
-    // 8*(e/4) + 4*(l/32) + e%4, column l%32.
     const uint32_t qq = q * q;
 #pragma unroll
-    for (int x = 0; x < 2; x++) {
+    for (int x = 0; x < 4; x++) {
 #pragma unroll
-        for (int y = 0; y < 2; y++) {
-            uint32_t c = cb * 64 + y * 32 + lr;
+        for (int y = 0; y < 4; y++) {
+            const uint32_t c = cb * 128 + y * 32 + lr;
             if (c >= Lq) continue;
-            uint32_t jsite = c / q, b = c - jsite * q;
+            const uint32_t jsite = c / q, b = c - jsite * q;
 #pragma unroll
             for (int e = 0; e < 16; e++) {
-                uint32_t r = rb * 64 + x * 32 + 8 * (e / 4) + 4 * g + (e % 4);
+                const uint32_t r = rb * 128 + x * 32 + 8 * (e / 4) + 4 * g + (e % 4);
                 if (r >= Lq) continue;
-                uint32_t isite = r / q, a = r - isite * q;
-                if (isite < jsite)
-                    counts[(size_t)pair_index(isite, jsite, L) * qq + a * q + b] = (uint32_t)acc[x][y][e];
+                const uint32_t isite = r / q, a = r - isite * q;
+                if (isite < jsite) {
+                    uint32_t* dst = counts + (size_t)pair_index(isite, jsite, L) * qq + a * q + b;
+                    if (ksplit == 1)
+                        *dst = (uint32_t)acc[x][y][e];
+                    else
+                        atomicAdd(dst, (uint32_t)acc[x][y][e]);
+                }
             }
         }
     }

@@ -610,12 +610,25 @@ int mi3_calc_bicounts(mi3_ctx* c, const char* seqbuf, uint32_t nseq) {
     if (ld % 16) return fail(c, MI3_EINVAL, "sequence buffer row stride must be a multiple of 16");
     rc = need(c, "bicount", (size_t)c->nP * c->q * c->q * 4, &cb);
     if (rc) return rc;
-    uint32_t Lq = c->L * c->q;
-    uint32_t nblk = (Lq + 63) / 64;
-    uint32_t nwav = nblk * (nblk + 1) / 2;
+    const uint32_t Lq = c->L * c->q;
+    const uint32_t nblk = (Lq + 127) / 128;
+    const uint32_t nbw = nblk * (nblk + 1) / 2;
+    // split the sequences so that about 2048 waves fill the chip, keeping at
+    // least 64 k-steps of 32 sequences per wave
+    const uint32_t ksteps = (nseq + 31) / 32;
+    uint32_t ksplit = (2048 + nbw - 1) / nbw;
+    const uint32_t kmax = ksteps / 64 > 1 ? ksteps / 64 : 1;
+    if (ksplit > kmax) ksplit = kmax;
+    if (ksplit > 16) ksplit = 16;
+    if (ksplit < 1) ksplit = 1;
     Scope s(c, "calcBicounts", (size_t)nseq * c->L);
-    hipLaunchKernelGGL(k_bicounts_mfma, dim3(nwav), dim3(64), 0, c->stream, (const uint8_t*)sb->d,
-                       (uint32_t)ld, nseq, c->L, c->q, nblk, (uint32_t*)cb->d);
+    if (ksplit > 1) {
+        int frc = fill_u32_dev(c, cb->d, 0u, (uint64_t)c->nP * c->q * c->q);
+        if (frc) return frc;
+    }
+    hipLaunchKernelGGL(k_bicounts_mfma128, dim3(nbw * ksplit), dim3(64), 0, c->stream,
+                       (const uint8_t*)sb->d, (uint32_t)ld, nseq, c->L, c->q, nblk, ksplit,
+                       (uint32_t*)cb->d);
     HIPCHK(c, hipGetLastError());
     cb->version++;
     return MI3_OK;
