@@ -56,8 +56,19 @@ __device__ __forceinline__ v4i onehot16(const uint8_t* __restrict__ seq, uint32_
 // operand built from a 16-byte load feeds 4 matrix instructions.  The
 // sequence range is split over the ksplit waves of a block, whose partial
 // counts are added with integer atomics (exact in any order; the table must
-// be zeroed first).  The next k-step's residue bytes are loaded before the
-// current step's products.
+// be zeroed first).  Residue bytes are loaded two k-steps ahead of use.
+//
+// Operands: row operands are -128 (0x80) where the residue matches, column
+// operands +1, so each co-occurrence adds -128 and count = -acc / 128
+// (|acc| <= 128 * nseq < 2^31 for nseq < 2^24).  With residues < 32 every
+// byte of x = v ^ a is below 0x80, so 0x80808080 - x borrows nothing across
+// bytes and its top bit per byte is set exactly where v == a; it is formed
+// in one v_xad_u32 as (v ^ ~a) + 0x80808081.  Rows past L*q compare against
+// 0x7f, which no residue equals.
+__device__ __forceinline__ uint32_t match_hi(uint32_t v, uint32_t na4) {
+    return (v ^ na4) + 0x80808081u;
+}
+
 __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restrict__ seq,
                                                          uint32_t ldseq, uint32_t nseq,
                                                          uint32_t L, uint32_t q, uint32_t nblk,
@@ -78,20 +89,22 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
     const uint32_t nsteps = (nseq + 31) / 32;
     const uint32_t s0 = (uint32_t)(((uint64_t)nsteps * part) / ksplit);
     const uint32_t s1 = (uint32_t)(((uint64_t)nsteps * (part + 1)) / ksplit);
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(seq), (short)0,
+                                          (int)(ldseq * L), 0x00020000);
 
-    uint32_t offA[4], a4A[4], offB[4], a4B[4];
-    bool vA[4], vB[4];
+    uint32_t offA[4], a4A[4], naA[4], offB[4], a4B[4], naB[4];
 #pragma unroll
     for (int x = 0; x < 4; x++) {
         const uint32_t r = rb * 128 + x * 32 + lr;
-        vA[x] = r < Lq;
-        const uint32_t si = vA[x] ? r / q : 0;
-        a4A[x] = (r - si * q) * 0x01010101u;
+        const uint32_t si = r < Lq ? r / q : 0;
+        a4A[x] = r < Lq ? (r - si * q) * 0x01010101u : 0x7f7f7f7fu;
+        naA[x] = ~a4A[x];
         offA[x] = si * ldseq + 16 * g;
         const uint32_t c = cb * 128 + x * 32 + lr;
-        vB[x] = c < Lq;
-        const uint32_t sj = vB[x] ? c / q : 0;
-        a4B[x] = (c - sj * q) * 0x01010101u;
+        const uint32_t sj = c < Lq ? c / q : 0;
+        a4B[x] = c < Lq ? (c - sj * q) * 0x01010101u : 0x7f7f7f7fu;
+        naB[x] = ~a4B[x];
         offB[x] = sj * ldseq + 16 * g;
     }
     v16i acc[4][4];
@@ -102,52 +115,60 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[x][y][e] = 0;
 
-    auto load = [&](uint32_t step, uint4 (&ra)[4], uint4 (&rbv)[4]) {
-        const uint32_t n = step * 32;
+    auto load = [&](uint32_t step, v4i (&ra)[4], v4i (&rbv)[4]) {
+        const uint32_t n = step * 32;   // wave-uniform: the scalar offset
 #pragma unroll
         for (int x = 0; x < 4; x++) {
-            ra[x] = vA[x] ? *reinterpret_cast<const uint4*>(seq + offA[x] + n) : make_uint4(0, 0, 0, 0);
-            rbv[x] = vB[x] ? *reinterpret_cast<const uint4*>(seq + offB[x] + n) : make_uint4(0, 0, 0, 0);
+            ra[x] = __builtin_amdgcn_raw_buffer_load_b128(rs, offA[x], n, 0);
+            rbv[x] = __builtin_amdgcn_raw_buffer_load_b128(rs, offB[x], n, 0);
         }
     };
-    auto onehot = [](const uint4& v, uint32_t a4, bool valid) {
-        v4i r = {0, 0, 0, 0};
-        if (valid) {
-            r[0] = (int)eq_bytes(v.x, a4);
-            r[1] = (int)eq_bytes(v.y, a4);
-            r[2] = (int)eq_bytes(v.z, a4);
-            r[3] = (int)eq_bytes(v.w, a4);
-        }
-        return r;
-    };
-    // full steps (all 32 sequences < nseq); the ragged last step, if any,
-    // goes through the byte-wise path
-    const uint32_t full_end = nseq / 32;
-    const uint32_t e1 = s1 < full_end ? s1 : full_end;
-    uint4 ra[4], rbv[4];
-    if (s0 < e1) load(s0, ra, rbv);
-    for (uint32_t st = s0; st < e1; st++) {
-        v4i A[4], B[4];
+    auto operands = [&](const v4i (&ra)[4], const v4i (&rbv)[4], v4i (&A)[4], v4i (&B)[4]) {
 #pragma unroll
-        for (int x = 0; x < 4; x++) {
-            A[x] = onehot(ra[x], a4A[x], vA[x]);
-            B[x] = onehot(rbv[x], a4B[x], vB[x]);
-        }
-        if (st + 1 < e1) load(st + 1, ra, rbv);
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                A[x][e] = (int)(match_hi((uint32_t)ra[x][e], naA[x]) & 0x80808080u);
+                B[x][e] = (int)((match_hi((uint32_t)rbv[x][e], naB[x]) >> 7) & 0x01010101u);
+            }
+    };
+    auto products = [&](const v4i (&A)[4], const v4i (&B)[4]) {
 #pragma unroll
         for (int x = 0; x < 4; x++)
 #pragma unroll
             for (int y = 0; y < 4; y++)
                 acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[x], B[y], acc[x][y], 0, 0, 0);
+    };
+    // full steps (all 32 sequences < nseq), two buffers of raw bytes loaded
+    // two steps ahead; the ragged last step, if any, goes byte by byte
+    const uint32_t full_end = nseq / 32;
+    const uint32_t e1 = s1 < full_end ? s1 : full_end;
+    v4i r0a[4], r0b[4], r1a[4], r1b[4], A[4], B[4];
+    if (s0 < e1) load(s0, r0a, r0b);
+    if (s0 + 1 < e1) load(s0 + 1, r1a, r1b);
+    for (uint32_t st = s0; st < e1; st += 2) {
+        operands(r0a, r0b, A, B);
+        if (st + 2 < e1) load(st + 2, r0a, r0b);
+        products(A, B);
+        if (st + 1 < e1) {
+            operands(r1a, r1b, A, B);
+            if (st + 3 < e1) load(st + 3, r1a, r1b);
+            products(A, B);
+        }
     }
     if (s1 > full_end && full_end >= s0) {
-        // ragged tail: the last, partial step of this part
+        // ragged tail: the last, partial step of this part (bytes past nseq
+        // count for nothing)
         const uint32_t n = full_end * 32 + 16 * g;
-        v4i A[4], B[4];
 #pragma unroll
         for (int x = 0; x < 4; x++) {
-            A[x] = onehot16(seq, ldseq, offA[x] / ldseq, a4A[x], vA[x], n, nseq);
-            B[x] = onehot16(seq, ldseq, offB[x] / ldseq, a4B[x], vB[x], n, nseq);
+            v4i ra = onehot16(seq, ldseq, offA[x] / ldseq, a4A[x], a4A[x] != 0x7f7f7f7fu, n, nseq);
+            v4i rbv = onehot16(seq, ldseq, offB[x] / ldseq, a4B[x], a4B[x] != 0x7f7f7f7fu, n, nseq);
+            // onehot16 gives 0/1 bytes; rows take -128
+#pragma unroll
+            for (int e = 0; e < 4; e++) ra[e] = (int)((uint32_t)ra[e] << 7);
+            A[x] = ra;
+            B[x] = rbv;
         }
 #pragma unroll
         for (int x = 0; x < 4; x++)
@@ -170,11 +191,12 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
                 if (r >= Lq) continue;
                 const uint32_t isite = r / q, a = r - isite * q;
                 if (isite < jsite) {
+                    const uint32_t cnt = (0u - (uint32_t)acc[x][y][e]) >> 7;
                     uint32_t* dst = counts + (size_t)pair_index(isite, jsite, L) * qq + a * q + b;
                     if (ksplit == 1)
-                        *dst = (uint32_t)acc[x][y][e];
+                        *dst = cnt;
                     else
-                        atomicAdd(dst, (uint32_t)acc[x][y][e]);
+                        atomicAdd(dst, cnt);
                 }
             }
         }

@@ -81,6 +81,7 @@ struct mi3_ctx {
     uint32_t* pos = nullptr;
     size_t pos_cap = 0;
     uint32_t wpb = 0;  // sweep walkers per work-group (0 = default)
+    uint32_t ncu = 256;  // compute units of the device
     // derived device data
     std::map<std::string, Chunked> chunked;  // chunked layouts keyed by source buffer
     double* red = nullptr;                    // reduction partials
@@ -251,6 +252,11 @@ int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t s
     if (hipSetDevice(device) != hipSuccess) return fail(nullptr, MI3_EHIP, "hipSetDevice failed");
     mi3_ctx* c = new mi3_ctx;
     c->dev = device;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c->ncu = (uint32_t)prop.multiProcessorCount;
+    }
     c->L = L;
     c->q = q;
     c->nP = L * (L - 1) / 2;
@@ -613,13 +619,15 @@ int mi3_calc_bicounts(mi3_ctx* c, const char* seqbuf, uint32_t nseq) {
     const uint32_t Lq = c->L * c->q;
     const uint32_t nblk = (Lq + 127) / 128;
     const uint32_t nbw = nblk * (nblk + 1) / 2;
-    // split the sequences so that about 2048 waves fill the chip, keeping at
-    // least 64 k-steps of 32 sequences per wave
+    // split the sequences so that about four waves per SIMD (one wave fits
+    // per SIMD at a time) share the work evenly, keeping at least 64 k-steps
+    // of 32 sequences per wave
     const uint32_t ksteps = (nseq + 31) / 32;
-    uint32_t ksplit = (2048 + nbw - 1) / nbw;
+    const uint32_t target = 4 * 4 * c->ncu;
+    uint32_t ksplit = (target + nbw / 2) / nbw;
     const uint32_t kmax = ksteps / 64 > 1 ? ksteps / 64 : 1;
     if (ksplit > kmax) ksplit = kmax;
-    if (ksplit > 16) ksplit = 16;
+    if (ksplit > 32) ksplit = 32;
     if (ksplit < 1) ksplit = 1;
     Scope s(c, "calcBicounts", (size_t)nseq * c->L);
     if (ksplit > 1) {
