@@ -101,6 +101,15 @@ int mi3_ticket_wait(mi3_ctx *ctx, int ticket, void *dst, size_t nbytes);
  * fill kernel, so it is safe inside captured graphs. */
 int mi3_buf_fill_u32(mi3_ctx *ctx, const char *name, uint32_t pattern);
 
+/* Device address of a named buffer and the context's HIP stream, for
+ * callers that run their own kernels on the engine's memory (the learned
+ * proposal's training step).  After writing a buffer through its address,
+ * call mi3_buf_mark_written so derived caches (the chunked coupling copy)
+ * are rebuilt.  No counterpart in the reference. */
+int mi3_buf_device_ptr(mi3_ctx *ctx, const char *name, void **ptr);
+int mi3_buf_mark_written(mi3_ctx *ctx, const char *name);
+int mi3_stream(mi3_ctx *ctx, void **stream);
+
 /* Wait for all work on the context's stream (MCMCGPU.wait, mcmcGPU.py:1016). */
 int mi3_synchronize(mi3_ctx *ctx);
 
@@ -275,6 +284,75 @@ int mi3_batch_set_tile(mi3_batch *b, uint32_t model, uint32_t wpb);
 
 /* Global step counter of a model's random streams. */
 int mi3_batch_step_count(mi3_batch *b, uint32_t model, uint64_t *steps);
+
+/* ---- learned proposal (masked autoregressive MLP) -------------------------
+ * Kernels of a MADE network (Germain et al. 2015) over sequence positions,
+ * used as an independence Metropolis-Hastings proposal beside the sweep:
+ *   h1 = relu(b1 + sum_i W1[:, q*i + s_i]), h2 = relu(W2 h1 + b2),
+ *   logits = W3 h2 + b3, p(s_j | s_<j) = softmax of site j's q logits,
+ * with the MADE masks applied to the weights by the caller.  Launchers take
+ * raw device pointers and a HIP stream (NULL = default stream); sequences
+ * are site-major (L rows of ld bytes); bf16 operands are raw 16-bit arrays.
+ * No counterpart in the reference.  Errors: mi3_made_last_error(). */
+const char *mi3_made_last_error(void);
+
+/* C = alpha*op(A)op(B) + beta*C + bias[col], relu if set; fp32 C and/or
+ * bf16 Cb.  op(A) is M x K (A is K x M row-major if transA), op(B) is K x N
+ * (B is N x K if transB).  M, N, K, lda, ldb multiples of 8. */
+int mi3_gemm_bf16(void *stream, int transA, int transB, int M, int N, int K, float alpha,
+                  const void *A, int lda, const void *B, int ldb, float beta, float *C, int ldc,
+                  void *Cb, int ldcb, const float *bias, int relu);
+
+/* h1 = b1 + sum_i W1t[q*i + s_i][:] (W1t: (L*q, H) float32), h1b = bf16(relu(h1)). */
+int mi3_made_embed(void *stream, const uint8_t *seq, uint32_t ld, uint32_t B, uint32_t L,
+                   uint32_t q, uint32_t H, const float *W1t, const float *b1, float *h1,
+                   void *h1b);
+
+/* dW1t[q*i + a][:] = sum of dh1[b][:] over the b with s_i(b) == a (every
+ * entry written). */
+int mi3_made_embed_grad(void *stream, const uint8_t *seq, uint32_t ld, uint32_t B, uint32_t L,
+                        uint32_t q, uint32_t H, const float *dh1, float *dW1t);
+
+/* nll[b] = -sum_j log softmax(logits[b, site j])[s_j]; optional gradient
+ * gscale*(softmax - onehot) as bf16 and/or float32 (B x L*q). */
+int mi3_made_xent(void *stream, const float *logits, const uint8_t *seq, uint32_t ld, uint32_t B,
+                  uint32_t L, uint32_t q, float gscale, float *nll, void *dlogits_bf16,
+                  float *dlogits);
+
+/* out = dh * (hpre > 0), float32 and/or bf16. */
+int mi3_relu_grad(void *stream, uint64_t n, const float *dh, const float *hpre, float *out,
+                  void *outb);
+
+/* Autoregressive sampling: h1 = b1 at the start; for each site j, with
+ * h2b = bf16(relu(W2 relu(h1) + b2)) from mi3_gemm_bf16, mi3_made_site
+ * draws s_j (u from Philox((j, walker_base+b, counter, 'MADE')) keyed by
+ * seed), writes it to row j of sout and accumulates -log p(s_j) in nll
+ * (written at j = 0); mi3_made_h1_update then adds W1t[q*j + s_j] to h1. */
+int mi3_made_h1_init(void *stream, uint32_t B, uint32_t H, const float *b1, float *h1, void *h1b);
+int mi3_made_site(void *stream, uint32_t j, const void *h2b, uint32_t H, const void *W3b,
+                  const float *b3, uint32_t q, uint32_t B, uint64_t seed, uint32_t counter,
+                  uint32_t walker_base, uint8_t *sout, uint32_t ld, float *nll);
+int mi3_made_h1_update(void *stream, uint32_t j, const uint8_t *sout, uint32_t ld, uint32_t B,
+                       uint32_t q, uint32_t H, const float *W1t, float *h1, void *h1b);
+
+/* Independence Metropolis-Hastings step: walker w takes its proposal with
+ * probability min(1, exp(-(Eprop - Ecur) - nll_cur + nll_prop)), where
+ * nll = -log q under the proposal, u from Philox((counter, walker_base+w,
+ * 'MHAC')); accepted columns, energies and nll are copied; naccept counts
+ * the moves. */
+int mi3_mh_accept(void *stream, uint32_t N, uint32_t L, uint8_t *cur, uint32_t ldc,
+                  const uint8_t *prop, uint32_t ldp, float *Ecur, const float *Eprop,
+                  float *nll_cur, const float *nll_prop, uint64_t seed, uint64_t counter,
+                  uint32_t walker_base, uint32_t *naccept);
+
+/* Training helpers: one Adam step (moments m, v; gradient times the 0/1
+ * mask if given; the updated parameters masked again and copied to bf16 pb
+ * if given), column sums of a rows x cols float32 matrix, float32 -> bf16. */
+int mi3_adam_step(void *stream, uint64_t n, float *p, const float *g, float *m, float *v,
+                  const float *mask, float lr, float beta1, float beta2, float eps, uint32_t t,
+                  void *pb);
+int mi3_colsum(void *stream, uint32_t rows, uint32_t cols, const float *x, float *out);
+int mi3_to_bf16(void *stream, uint64_t n, const float *p, void *pb);
 
 /* ---- profiling ------------------------------------------------------------ */
 

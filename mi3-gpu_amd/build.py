@@ -21,7 +21,7 @@ OUT = HERE / 'mi3gpu' / 'libmi3hip.so'
 BUILD = HERE / '_build'
 ARCH = os.environ.get('MI3_OFFLOAD_ARCH', 'gfx950')
 
-UNITS = ['runtime.hip', 'potts.hip', 'batch.hip']
+UNITS = ['runtime.hip', 'potts.hip', 'batch.hip', 'made.hip']
 
 CFLAGS = [
     f'--offload-arch={ARCH}', '-O3', '-fPIC', '-std=c++17',

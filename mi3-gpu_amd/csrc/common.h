@@ -9,6 +9,8 @@ namespace mi3 {
 constexpr uint32_t TAG_WALK = 0x57414C4Bu;   // per-walker proposal / acceptance draws
 constexpr uint32_t TAG_POS = 0x504F5349u;    // shared site sequence
 constexpr uint32_t TAG_INDEP = 0x494E4450u;  // independent-site reseed
+constexpr uint32_t TAG_MADE = 0x4D414445u;   // autoregressive proposal draws
+constexpr uint32_t TAG_MHAC = 0x4D484143u;   // independence Metropolis-Hastings accepts
 
 // Sites per LDS chunk and lanes per walker group in the sweep/energy kernels.
 constexpr int CH = 32;

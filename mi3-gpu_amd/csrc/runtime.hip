@@ -337,6 +337,28 @@ int mi3_buf_alloc(mi3_ctx* c, const char* name, size_t nbytes) {
 
 int mi3_buf_exists(mi3_ctx* c, const char* name) { return (c && find(c, name)) ? 1 : 0; }
 
+int mi3_buf_device_ptr(mi3_ctx* c, const char* name, void** ptr) {
+    if (!c || !ptr) return MI3_EINVAL;
+    Buf* b = find(c, name);
+    if (!b) return fail(c, MI3_ENOBUF, std::string("no buffer '") + (name ? name : "(null)") + "'");
+    *ptr = b->d;
+    return MI3_OK;
+}
+
+int mi3_buf_mark_written(mi3_ctx* c, const char* name) {
+    if (!c) return MI3_EINVAL;
+    Buf* b = find(c, name);
+    if (!b) return fail(c, MI3_ENOBUF, std::string("no buffer '") + (name ? name : "(null)") + "'");
+    b->version++;
+    return MI3_OK;
+}
+
+int mi3_stream(mi3_ctx* c, void** stream) {
+    if (!c || !stream) return MI3_EINVAL;
+    *stream = c->stream;
+    return MI3_OK;
+}
+
 int mi3_buf_size(mi3_ctx* c, const char* name, size_t* nbytes) {
     Buf* b = c ? find(c, name) : nullptr;
     if (!b) return c ? fail(c, MI3_ENOBUF, "no such buffer") : MI3_EINVAL;

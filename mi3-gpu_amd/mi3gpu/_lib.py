@@ -60,6 +60,25 @@ SIGNATURES = {
     'mi3_reg_l2z': (_i, [_P, _f, _f, _f]),
     'mi3_update_J_reg': (_i, [_P, _f, _f, _i, _f]),
     'mi3_copy_seqs': (_i, [_P, _S, _u32, _S, _u32, _u32]),
+    'mi3_buf_device_ptr': (_i, [_P, _S, _c.POINTER(_P)]),
+    'mi3_buf_mark_written': (_i, [_P, _S]),
+    'mi3_stream': (_i, [_P, _c.POINTER(_P)]),
+    'mi3_made_last_error': (_S, []),
+    'mi3_gemm_bf16': (_i, [_P, _i, _i, _i, _i, _i, _f, _P, _i, _P, _i, _f, _P, _i, _P, _i, _P,
+                           _i]),
+    'mi3_made_embed': (_i, [_P, _P, _u32, _u32, _u32, _u32, _u32, _P, _P, _P, _P]),
+    'mi3_made_embed_grad': (_i, [_P, _P, _u32, _u32, _u32, _u32, _u32, _P, _P]),
+    'mi3_made_xent': (_i, [_P, _P, _P, _u32, _u32, _u32, _u32, _f, _P, _P, _P]),
+    'mi3_relu_grad': (_i, [_P, _u64, _P, _P, _P, _P]),
+    'mi3_made_h1_init': (_i, [_P, _u32, _u32, _P, _P, _P]),
+    'mi3_made_site': (_i, [_P, _u32, _P, _u32, _P, _P, _u32, _u32, _u64, _u32, _u32, _P, _u32,
+                           _P]),
+    'mi3_made_h1_update': (_i, [_P, _u32, _P, _u32, _u32, _u32, _u32, _P, _P, _P]),
+    'mi3_adam_step': (_i, [_P, _u64, _P, _P, _P, _P, _P, _f, _f, _f, _f, _u32, _P]),
+    'mi3_colsum': (_i, [_P, _u32, _u32, _P, _P]),
+    'mi3_to_bf16': (_i, [_P, _u64, _P, _P]),
+    'mi3_mh_accept': (_i, [_P, _u32, _u32, _P, _u32, _P, _u32, _P, _P, _P, _P, _u64, _u64, _u32,
+                           _P]),
     'mi3_batch_create': (_i, [_i, _u32, _c.POINTER(_u32), _c.POINTER(_u32), _c.POINTER(_u32),
                               _u64, _c.POINTER(_P)]),
     'mi3_batch_destroy': (_i, [_P]),
