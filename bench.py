@@ -235,6 +235,127 @@ def measure_kernels(n=2 ** 20, L=232, q=21, reps=3):
             "energies_ms": en_ms, "energy_gathers_per_s": float(n) * nP / (en_ms * 1e-3)}
 
 
+def _best_wgsize(J, s0, L, q, seed):
+    """Walkers per work-group with the fastest single sweep for this walker
+    count (128 / 256 / 512), from three timed launches each."""
+    from mi3gpu.mcmcGPU import MCMCGPU
+    best = None
+    for wg in (128, 256, 512):
+        with tempfile.TemporaryDirectory() as d:
+            g = MCMCGPU((0, 0, None, None), L, q, len(s0), wg, Path(d), 1024, seed)
+            g.initMCMC(L, 0, 2 ** 63)
+            g.setBuf('J', J)
+            g.setBuf('seq main', s0)
+            g.runMCMC()
+            g.wait()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                g.runMCMC()
+            g.wait()
+            dt = time.perf_counter() - t0
+            del g
+        if best is None or dt < best[1]:
+            best = (wg, dt)
+    return best[0]
+
+
+def measure_made(J, seqs, L, q, ntrain=2 ** 18, N=2 ** 16, H=512, train_steps=1500,
+                 batch=4096, lr=1e-3, ratio=8, plain_sweeps=128, cycles=48, timed=8, seed=7):
+    """Learned proposal against the plain sweep on the same GPU and walkers.
+
+    A MADE (L*q -> H -> H -> L*q, bf16 matrix-core products) is trained by
+    Adam on `ntrain` equilibrated MCMC sequences of the model; N walkers
+    (other equilibrated sequences) then run cycles of `ratio` sweeps followed
+    by one independence Metropolis-Hastings step with MADE proposals.  Both
+    forms are timed by wall clock over whole cycles (sweeps, energies,
+    proposals), and their energy traces (per sweep for the plain form, per
+    cycle for the mixed one) give the integrated autocorrelation time;
+    energy-ESS/s = N / (2 tau t_interval)."""
+    from mi3gpu.mcmcGPU import MCMCGPU
+    from mi3gpu.made import MADE, MixedSampler
+    from mi3gpu.utils.ess import energy_ess
+    train, s0 = seqs[:ntrain], seqs[ntrain:ntrain + N]
+    wg = _best_wgsize(J, s0, L, q, seed)
+    f1 = np.stack([np.bincount(train[:, i], minlength=q) for i in range(L)]) / len(train)
+    made = MADE(L, q, H, max_batch=N, seed=seed, unimarg=f1)
+    made.mem.alloc('train', (L, ntrain), 'u1')
+    made.mem.write('train', np.ascontiguousarray(train.T))
+    made.mem.alloc('heldout', (L, N), 'u1')
+    made.mem.write('heldout', np.ascontiguousarray(s0.T))
+    made.mem.alloc('heldout nll', (N,), 'f4')
+    made.mem.sync()
+    t0 = time.perf_counter()
+    hist = made.train(made.mem['train'], ntrain, ntrain, steps=train_steps, batch=batch, lr=lr,
+                      seed=seed, log_every=max(1, train_steps // 6))
+    t_train = time.perf_counter() - t0
+    print(f"MADE trained: {train_steps} steps in {t_train:.1f} s, nll {hist[-1][1]:.3f}",
+          file=sys.stderr, flush=True)
+    made.nll(made.mem['heldout'], N, N, made.mem['heldout nll'])
+    heldout = float(np.mean(made.get('heldout nll')))
+
+    def run(mixed, T):
+        with tempfile.TemporaryDirectory() as d:
+            g = MCMCGPU((0, 0, None, None), L, q, N, wg, Path(d), 1024, seed + 1)
+            g.initMCMC(L, 0, 2 ** 63)
+            g.setBuf('J', J)
+            g.setBuf('seq main', s0)
+            ms = MixedSampler(g, made, seed=seed + 2) if mixed else None
+
+            def interval():
+                if mixed:
+                    for _ in range(ratio):
+                        g.runMCMC()
+                    ms.propose()
+                else:
+                    g.runMCMC()
+                    g.calcEnergies('main')
+
+            interval()
+            g.wait()
+            n_timed = timed if mixed else timed * ratio
+            t0 = time.perf_counter()
+            for _ in range(n_timed):
+                interval()
+            g.wait()
+            t_int = (time.perf_counter() - t0) / n_timed
+            E = np.empty((T, N), np.float32)
+            for t in range(T):
+                interval()
+                E[t] = g.getBuf('E main').read()
+            acc = ms.accepted() / (N * (T + n_timed + 1)) if mixed else None
+            if mixed:   # cost of the proposal step alone
+                g.wait()
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    ms.propose()
+                g.wait()
+                t_prop = (time.perf_counter() - t0) / 3
+            del g
+        ess, tau, W = energy_ess(E)
+        r = {"energy_ess_per_s": N / (2 * tau * t_int), "tau_int": tau, "window": W,
+             "interval_ms": t_int * 1e3, "traced_intervals": T,
+             "mean_E": float(E.mean()), "sem_E": float(E.mean(axis=0).std() / np.sqrt(N) *
+                                                       np.sqrt(2 * tau))}
+        if mixed:
+            r.update(acceptance=acc, proposal_step_ms=t_prop * 1e3)
+        return r
+
+    plain = run(False, plain_sweeps)
+    print(f"plain: tau {plain['tau_int']:.2f} sweeps", file=sys.stderr, flush=True)
+    mixed = run(True, cycles)
+    print(f"mixed: tau {mixed['tau_int']:.2f} cycles, acceptance {mixed['acceptance']:.3f}",
+          file=sys.stderr, flush=True)
+    made.close()
+    return {"value": mixed["energy_ess_per_s"], "unit": "energy effective samples/s",
+            "vs_plain": mixed["energy_ess_per_s"] / plain["energy_ess_per_s"],
+            "plain": dict(plain, interval="1 sweep"),
+            "mixed": dict(mixed, interval=f"{ratio} sweeps + 1 MADE proposal per walker"),
+            "walkers": N, "walkers_per_workgroup": wg,
+            "made": {"H": H, "train_sequences": ntrain, "train_steps": train_steps,
+                     "batch": batch, "lr": lr, "train_s": t_train,
+                     "nll_history": hist, "heldout_nll": heldout}}
+
+
 def measure_config1(seed=0):
     """The reference's CPU-runnable round: L=99, q=4, h ~ N(0,0.5),
     J ~ N(0,0.1) (zero-sum gauge, fieldless float32); target bimarginals of
@@ -335,6 +456,10 @@ def main():
                     help="also time bicounts and energies standalone on 2^20 sequences")
     ap.add_argument('--batch-rounds', type=int, default=0,
                     help="also time the batched uneven-models workload (rounds)")
+    ap.add_argument('--made', action='store_true',
+                    help="also measure the learned proposal (MADE mixed 1:8 with sweeps, "
+                         "2^16 walkers) against the plain sweep")
+    ap.add_argument('--made-train-steps', type=int, default=1500)
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help="also run 3 device-to-device copies of MIB MiB (the reference "
                          "kernel that scales HBM byte counters)")
@@ -401,10 +526,13 @@ def main():
                      "kernel": "k_sweep", "kernel_ms": kms, "launches": kcnt,
                      "alg_bytes_per_launch": alg_bytes},
     }
-    if a.ess_sweeps > 0 or a.rounds > 0:
+    if a.ess_sweeps > 0 or a.rounds > 0 or a.made:
         tseqs, target = sample_target(J, L, q, 2 ** 20, a.target_sweeps, 4, a.wgsize)
         if a.ess_sweeps > 0:
             out["energy_ess"] = measure_ess(J, tseqs[:N], L, q, a.ess_sweeps, 5, a.wgsize)
+        if a.made:
+            out["learned_proposal"] = measure_made(J, tseqs, L, q,
+                                                   train_steps=a.made_train_steps)
         del tseqs
         if a.rounds > 0:
             out["round"] = measure_rounds(target, L, q, N, a.rounds, 6)
