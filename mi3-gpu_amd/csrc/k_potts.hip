@@ -172,6 +172,32 @@ __device__ __forceinline__ uint32_t select_word_opaque(const uint32_t (&w)[KW], 
     return r;
 }
 
+// 128 * byte e of w (e a compile-time constant after unrolling) in one
+// VALU op: a shift with an SDWA byte select on its source.  Left to itself
+// the compiler makes byte 0 a shift and a mask.
+__device__ __forceinline__ uint32_t byte_x128(uint32_t w, int e) {
+    uint32_t r;
+    switch (e) {
+        case 0:
+            asm("v_lshlrev_b32_sdwa %0, 7, %1 dst_sel:DWORD dst_unused:UNUSED_PAD "
+                "src0_sel:DWORD src1_sel:BYTE_0" : "=v"(r) : "v"(w));
+            break;
+        case 1:
+            asm("v_lshlrev_b32_sdwa %0, 7, %1 dst_sel:DWORD dst_unused:UNUSED_PAD "
+                "src0_sel:DWORD src1_sel:BYTE_1" : "=v"(r) : "v"(w));
+            break;
+        case 2:
+            asm("v_lshlrev_b32_sdwa %0, 7, %1 dst_sel:DWORD dst_unused:UNUSED_PAD "
+                "src0_sel:DWORD src1_sel:BYTE_2" : "=v"(r) : "v"(w));
+            break;
+        default:
+            asm("v_lshlrev_b32_sdwa %0, 7, %1 dst_sel:DWORD dst_unused:UNUSED_PAD "
+                "src0_sel:DWORD src1_sel:BYTE_3" : "=v"(r) : "v"(w));
+            break;
+    }
+    return r;
+}
+
 __device__ __forceinline__ uint32_t lane_select_u(uint64_t mask, uint32_t a, uint32_t b) {
     uint32_t r;
     asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(mask));
@@ -321,9 +347,9 @@ __device__ __forceinline__ void sweep_block(
 #pragma unroll
                     for (int g = 0; g < G; g++) {
                         const int s = s0 + g;
-                        uint32_t off =
-                            ((state_word<KW>(st[s], (uint32_t)k >> 2) >> ((k & 3) * 8)) & 0xffu) *
-                            (CH * 4);
+                        static_assert(CH * 4 == 128, "row stride of a chunk");
+                        const uint32_t off =
+                            byte_x128(state_word<KW>(st[s], (uint32_t)k >> 2), k & 3);
                         vA[g] = *reinterpret_cast<const float*>(cb + baseA[s] + off);
                         vB[g] = *reinterpret_cast<const float*>(cb + baseB[s] + off);
                     }
@@ -495,6 +521,15 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
                                           ldsb + (buf ^ 1) * chf * 4, wave);
                     }
                     const float* cb = lds + buf * chf;
+                    if constexpr (S > 16) {
+                        // the state is loop-invariant: without this the
+                        // compiler hoists the S*K extracted residues out of
+                        // the loops, which at 32 slots spills registers
+#pragma unroll
+                        for (int s = 0; s < S; s++)
+#pragma unroll
+                            for (int kw = 0; kw < KW; kw++) asm volatile("" : "+v"(st[s][kw]));
+                    }
                     // on the diagonal chunk only partners m > i count: a 0/1
                     // factor (fma(v, 1, acc) == acc + v exactly)
                     const float keep = ((uint32_t)k != kp || j > owner) ? 1.f : 0.f;

@@ -10,6 +10,11 @@ constexpr int NT = 64 * NWAVES;
 // walker slots per half-wave: fewer for long sequences so state stays in registers
 template <int KW>
 constexpr int slots() { return KW <= 2 ? 16 : 8; }
+// sequence slots per half-wave of the energy kernel: every staged coupling
+// chunk is read once per sequence, so more sequences per work-group cut the
+// L2 -> LDS traffic per sequence
+template <int KW>
+constexpr int eslots() { return KW <= 2 ? 32 : 16; }
 
 static_assert(NT == POTTS_THREADS, "thread count");
 
@@ -36,8 +41,8 @@ hipError_t sweep_kws(const SweepArgs& a, hipStream_t st) {
 
 template <int KW, int PF>
 hipError_t energies_kw(const EnergyArgs& a, hipStream_t st) {
-    constexpr uint32_t W = 2 * slots<KW>() * NWAVES;
-    auto kern = k_energies<KW, slots<KW>(), NWAVES, PF>;
+    constexpr uint32_t W = 2 * eslots<KW>() * NWAVES;
+    auto kern = k_energies<KW, eslots<KW>(), NWAVES, PF>;
     size_t lds = 2ull * chunk_floats(a.q) * sizeof(float);
     hipError_t e = set_lds(kern, lds);
     if (e != hipSuccess) return e;
