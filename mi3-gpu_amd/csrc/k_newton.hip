@@ -227,58 +227,81 @@ __global__ void k_weights_fixed(const float* __restrict__ w, uint32_t n,
 // Pass 2: one work-group per (row i, chunk of 32 partner sites m > i); the
 // 32 pair histograms live in LDS as uint64 [a*q+b][32] (lane j of a
 // half-wave owns pair (i, 32c+j): the adds of a half-wave hit distinct
-// banks).  Each of the 16 half-waves streams 16 sequences per iteration.
-__global__ __launch_bounds__(512) void k_weighted_marg(const uint8_t* __restrict__ seq,
-                                                       uint32_t ldseq, uint32_t nseq,
-                                                       const uint64_t* __restrict__ wq,
-                                                       const float* __restrict__ wsum,
-                                                       const uint32_t* __restrict__ bad,
-                                                       uint32_t L, uint32_t q,
-                                                       const uint32_t* __restrict__ units,
-                                                       float* __restrict__ out) {
+// banks).  Each of the WM_THREADS/32 half-waves takes groups of 16
+// sequences in turn and loads its next group before adding the current one,
+// so the global loads overlap the LDS atomics.
+constexpr uint32_t WM_THREADS = 1024;
+
+struct WmGroup {
+    uint4 si, sm;        // residues of sites i and m of 16 sequences
+    ulonglong2 w[8];     // their fixed-point weights
+};
+
+__device__ __forceinline__ void wm_load(WmGroup& g, const uint8_t* rowi, const uint8_t* rowm,
+                                        const uint64_t* wq, uint32_t n0) {
+    g.si = *reinterpret_cast<const uint4*>(rowi + n0);
+    g.sm = *reinterpret_cast<const uint4*>(rowm + n0);
+    const ulonglong2* w2 = reinterpret_cast<const ulonglong2*>(wq + n0);
+#pragma unroll
+    for (int k = 0; k < 8; k++) g.w[k] = w2[k];
+}
+
+__global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __restrict__ seq,
+                                                              uint32_t ldseq, uint32_t nseq,
+                                                              const uint64_t* __restrict__ wq,
+                                                              const float* __restrict__ wsum,
+                                                              const uint32_t* __restrict__ bad,
+                                                              uint32_t L, uint32_t q,
+                                                              const uint32_t* __restrict__ units,
+                                                              float* __restrict__ out) {
     extern __shared__ unsigned long long hist64[];
+    constexpr uint32_t NHW = WM_THREADS / 32;  // half-waves
     const uint32_t unit = units[blockIdx.x];
     const uint32_t i = unit >> 8, c = unit & 0xff;
     const uint32_t qq = q * q;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t e = tid; e < qq * CH; e += 512) hist64[e] = 0ull;
+    for (uint32_t e = tid; e < qq * CH; e += WM_THREADS) hist64[e] = 0ull;
     __syncthreads();
-    const uint32_t lane = tid & 63, j = lane & 31, hw = tid >> 5;  // 16 half-waves
+    const uint32_t lane = tid & 63, j = lane & 31, hw = tid >> 5;
     const uint32_t m = c * CH + j;
     const bool valid = (m > i) && (m < L);
     const uint32_t mrow = valid ? m : i;
     const uint8_t* rowi = seq + (size_t)i * ldseq;
     const uint8_t* rowm = seq + (size_t)mrow * ldseq;
     const uint32_t qch = q * CH;
-    for (uint32_t n0 = hw * 16; n0 < nseq; n0 += 16 * 16) {
-        if (n0 + 16 <= nseq) {
-            const uint4 si4 = *reinterpret_cast<const uint4*>(rowi + n0);
-            const uint4 sm4 = *reinterpret_cast<const uint4*>(rowm + n0);
-            const ulonglong2* w2 = reinterpret_cast<const ulonglong2*>(wq + n0);
-            ulonglong2 wv[8];
+    // full groups of 16 sequences: group g covers [16g, 16g + 16)
+    const uint32_t ngroups = nseq / 16;
+    uint32_t g = hw;
+    WmGroup cur;
+    if (g < ngroups) wm_load(cur, rowi, rowm, wq, g * 16);
+    while (g < ngroups) {
+        const uint32_t gn = g + NHW;
+        WmGroup nxt;
+        if (gn < ngroups) wm_load(nxt, rowi, rowm, wq, gn * 16);
+        if (valid) {
+            const uint32_t si[4] = {cur.si.x, cur.si.y, cur.si.z, cur.si.w};
+            const uint32_t sm[4] = {cur.sm.x, cur.sm.y, cur.sm.z, cur.sm.w};
 #pragma unroll
-            for (int k = 0; k < 8; k++) wv[k] = w2[k];
-            const uint32_t si[4] = {si4.x, si4.y, si4.z, si4.w};
-            const uint32_t sm[4] = {sm4.x, sm4.y, sm4.z, sm4.w};
-            if (valid) {
-#pragma unroll
-                for (int e = 0; e < 16; e++) {
-                    const uint32_t a = (si[e >> 2] >> (8 * (e & 3))) & 0xffu;
-                    const uint32_t b = (sm[e >> 2] >> (8 * (e & 3))) & 0xffu;
-                    const unsigned long long v = (e & 1) ? wv[e >> 1].y : wv[e >> 1].x;
-                    atomicAdd(&hist64[__umul24(a, qch) + b * CH + j], v);
-                }
+            for (int e = 0; e < 16; e++) {
+                const uint32_t a = (si[e >> 2] >> (8 * (e & 3))) & 0xffu;
+                const uint32_t b = (sm[e >> 2] >> (8 * (e & 3))) & 0xffu;
+                const unsigned long long v = (e & 1) ? cur.w[e >> 1].y : cur.w[e >> 1].x;
+                atomicAdd(&hist64[__umul24(a, qch) + b * CH + j], v);
             }
-        } else if (valid) {
-            for (uint32_t n = n0; n < nseq && n < n0 + 16; n++)
-                atomicAdd(&hist64[__umul24(rowi[n], qch) + rowm[n] * CH + j],
-                          (unsigned long long)wq[n]);
         }
+        cur = nxt;
+        g = gn;
+    }
+    // the last nseq % 16 sequences, by the first half-wave
+    if (hw == 0 && valid) {
+        for (uint32_t n = ngroups * 16; n < nseq; n++)
+            atomicAdd(&hist64[__umul24(rowi[n], qch) + rowm[n] * CH + j],
+                      (unsigned long long)wq[n]);
     }
     __syncthreads();
     // back to float: F = acc * sum(w) / 2^62, written once (packed order)
     const double back = bad[0] ? NAN : (double)wsum[0] * (1.0 / 4611686018427387904.0);
-    for (uint32_t e = tid; e < qq * CH; e += 512) {
+    for (uint32_t e = tid; e < qq * CH; e += WM_THREADS) {
         const uint32_t jj = e & 31, ac = e >> 5;
         const uint32_t mm = c * CH + jj;
         if (mm > i && mm < L)

@@ -712,7 +712,7 @@ int mi3_weighted_marg(mi3_ctx* c, const char* seqbuf, const char* wbuf, uint32_t
                        nb, 1, c->red_out);
     hipLaunchKernelGGL(k_weights_fixed, dim3(grid1(nseq ? nseq : 1)), dim3(256), 0, c->stream,
                        (const float*)wb->d, nseq, (const float*)c->red_out, c->wq, c->wbad);
-    hipLaunchKernelGGL(k_weighted_marg, dim3(c->n_units), dim3(512), lds, c->stream,
+    hipLaunchKernelGGL(k_weighted_marg, dim3(c->n_units), dim3(WM_THREADS), lds, c->stream,
                        (const uint8_t*)sb->d, (uint32_t)ld, nseq, (const uint64_t*)c->wq,
                        (const float*)c->red_out, (const uint32_t*)c->wbad, c->L, c->q,
                        (const uint32_t*)c->wm_units, (float*)bb->d);
