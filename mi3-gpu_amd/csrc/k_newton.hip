@@ -210,103 +210,108 @@ __global__ void k_add_floats(float* __restrict__ dst, const float* __restrict__ 
 // sum(w) * 2^-63, far under float32 summation error, and the result does
 // not depend on the order of the additions.
 //
-// Pass 1: fixed-point weights (a non-finite or zero total makes every output
-// NaN, which keeps the divergence check of the Newton loop working).
+// Pass 1: fixed-point weights wq = round(w * 2^31 / sum(w)), so that no bin
+// of any subset of sequences can exceed 2^31 (a non-finite or zero total
+// makes every output NaN, which keeps the divergence check of the Newton
+// loop working).
 __global__ void k_weights_fixed(const float* __restrict__ w, uint32_t n,
-                                const float* __restrict__ wsum, uint64_t* __restrict__ wq,
+                                const float* __restrict__ wsum, uint32_t* __restrict__ wq,
                                 uint32_t* __restrict__ bad) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const double tot = (double)wsum[0];
     const bool ok = tot > 0.0 && tot < INFINITY;
     if (i == 0) bad[0] = ok ? 0u : 1u;
     if (i >= n) return;
-    double v = ok ? (double)w[i] * (4611686018427387904.0 / tot) : 0.0;  // 2^62 / sum
-    wq[i] = (v > 0.0) ? (uint64_t)(v + 0.5) : 0ull;
+    double v = ok ? (double)w[i] * (2147483648.0 / tot) : 0.0;  // 2^31 / sum
+    wq[i] = (v > 0.0) ? (uint32_t)(v + 0.5) : 0u;
 }
 
-// Pass 2: one work-group per (row i, chunk of 32 partner sites m > i); the
-// 32 pair histograms live in LDS as uint64 [a*q+b][32] (lane j of a
-// half-wave owns pair (i, 32c+j): the adds of a half-wave hit distinct
-// banks).  Each of the WM_THREADS/32 half-waves takes groups of 16
-// sequences in turn and loads its next group before adding the current one,
-// so the global loads overlap the LDS atomics.
+// Pass 2: one work-group per (row i, group of 64 partner sites m > i, part
+// of the sequence range).  The 64 pair histograms live in LDS as uint32
+// [a*q+b][64]: lane l of every wave owns pair (i, 64*cp + l), and all 64
+// lanes of a wave add the same sequence, so the row-i residues and the
+// weights are wave-uniform loads and one wave's adds move 256 bytes.  The
+// 16 waves take groups of 16 sequences in turn, loading the next group
+// before adding the current one.  The part's histograms are added into the
+// uint32 accumulator (zeroed beforehand) with integer atomics, so the
+// result does not depend on the order of any addition.
 constexpr uint32_t WM_THREADS = 1024;
-
-struct WmGroup {
-    uint4 si, sm;        // residues of sites i and m of 16 sequences
-    ulonglong2 w[8];     // their fixed-point weights
-};
-
-__device__ __forceinline__ void wm_load(WmGroup& g, const uint8_t* rowi, const uint8_t* rowm,
-                                        const uint64_t* wq, uint32_t n0) {
-    g.si = *reinterpret_cast<const uint4*>(rowi + n0);
-    g.sm = *reinterpret_cast<const uint4*>(rowm + n0);
-    const ulonglong2* w2 = reinterpret_cast<const ulonglong2*>(wq + n0);
-#pragma unroll
-    for (int k = 0; k < 8; k++) g.w[k] = w2[k];
-}
+constexpr uint32_t WM_COLS = 64;
 
 __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __restrict__ seq,
                                                               uint32_t ldseq, uint32_t nseq,
-                                                              const uint64_t* __restrict__ wq,
-                                                              const float* __restrict__ wsum,
-                                                              const uint32_t* __restrict__ bad,
+                                                              const uint32_t* __restrict__ wq,
                                                               uint32_t L, uint32_t q,
                                                               const uint32_t* __restrict__ units,
-                                                              float* __restrict__ out) {
-    extern __shared__ unsigned long long hist64[];
-    constexpr uint32_t NHW = WM_THREADS / 32;  // half-waves
-    const uint32_t unit = units[blockIdx.x];
-    const uint32_t i = unit >> 8, c = unit & 0xff;
+                                                              uint32_t nsplit,
+                                                              uint32_t* __restrict__ acc) {
+    extern __shared__ uint32_t hist32[];
+    constexpr uint32_t NW = WM_THREADS / 64;
+    const uint32_t unit = units[blockIdx.x / nsplit], part = blockIdx.x % nsplit;
+    const uint32_t i = unit >> 8, cp = unit & 0xff;
     const uint32_t qq = q * q;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t e = tid; e < qq * CH; e += WM_THREADS) hist64[e] = 0ull;
+    for (uint32_t e = tid; e < qq * WM_COLS; e += WM_THREADS) hist32[e] = 0u;
     __syncthreads();
-    const uint32_t lane = tid & 63, j = lane & 31, hw = tid >> 5;
-    const uint32_t m = c * CH + j;
+    const uint32_t lane = tid & 63;
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
+    const uint32_t m = cp * WM_COLS + lane;
     const bool valid = (m > i) && (m < L);
-    const uint32_t mrow = valid ? m : i;
     const uint8_t* rowi = seq + (size_t)i * ldseq;
-    const uint8_t* rowm = seq + (size_t)mrow * ldseq;
-    const uint32_t qch = q * CH;
-    // full groups of 16 sequences: group g covers [16g, 16g + 16)
+    const uint8_t* rowm = seq + (size_t)(valid ? m : i) * ldseq;
+    const uint32_t qcol = q * WM_COLS;
+    // this part's groups of 16 sequences
     const uint32_t ngroups = nseq / 16;
-    uint32_t g = hw;
-    WmGroup cur;
-    if (g < ngroups) wm_load(cur, rowi, rowm, wq, g * 16);
-    while (g < ngroups) {
-        const uint32_t gn = g + NHW;
-        WmGroup nxt;
-        if (gn < ngroups) wm_load(nxt, rowi, rowm, wq, gn * 16);
+    const uint32_t per = (ngroups + nsplit - 1) / nsplit;
+    const uint32_t g0 = part * per;
+    const uint32_t g1 = (g0 + per < ngroups) ? g0 + per : ngroups;
+    uint32_t g = g0 + wave;
+    uint4 sm_cur = make_uint4(0, 0, 0, 0);
+    if (g < g1) sm_cur = *reinterpret_cast<const uint4*>(rowm + g * 16);
+    while (g < g1) {
+        const uint32_t gn = g + NW;
+        uint4 sm_nxt = make_uint4(0, 0, 0, 0);
+        if (gn < g1) sm_nxt = *reinterpret_cast<const uint4*>(rowm + gn * 16);
+        const uint4 si4 = *reinterpret_cast<const uint4*>(rowi + g * 16);
+        const uint4* w4 = reinterpret_cast<const uint4*>(wq + g * 16);
+        const uint4 wv[4] = {w4[0], w4[1], w4[2], w4[3]};
         if (valid) {
-            const uint32_t si[4] = {cur.si.x, cur.si.y, cur.si.z, cur.si.w};
-            const uint32_t sm[4] = {cur.sm.x, cur.sm.y, cur.sm.z, cur.sm.w};
+            const uint32_t si[4] = {si4.x, si4.y, si4.z, si4.w};
+            const uint32_t sm[4] = {sm_cur.x, sm_cur.y, sm_cur.z, sm_cur.w};
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const uint32_t a = (si[e >> 2] >> (8 * (e & 3))) & 0xffu;
                 const uint32_t b = (sm[e >> 2] >> (8 * (e & 3))) & 0xffu;
-                const unsigned long long v = (e & 1) ? cur.w[e >> 1].y : cur.w[e >> 1].x;
-                atomicAdd(&hist64[__umul24(a, qch) + b * CH + j], v);
+                const uint4 w = wv[e >> 2];
+                const uint32_t v = (e & 3) == 0 ? w.x : (e & 3) == 1 ? w.y : (e & 3) == 2 ? w.z : w.w;
+                atomicAdd(&hist32[__umul24(a, qcol) + b * WM_COLS + lane], v);
             }
         }
-        cur = nxt;
+        sm_cur = sm_nxt;
         g = gn;
     }
-    // the last nseq % 16 sequences, by the first half-wave
-    if (hw == 0 && valid) {
+    // the last nseq % 16 sequences, by the first wave of the last part
+    if (part + 1 == nsplit && wave == 0 && valid) {
         for (uint32_t n = ngroups * 16; n < nseq; n++)
-            atomicAdd(&hist64[__umul24(rowi[n], qch) + rowm[n] * CH + j],
-                      (unsigned long long)wq[n]);
+            atomicAdd(&hist32[__umul24(rowi[n], qcol) + rowm[n] * WM_COLS + lane], wq[n]);
     }
     __syncthreads();
-    // back to float: F = acc * sum(w) / 2^62, written once (packed order)
-    const double back = bad[0] ? NAN : (double)wsum[0] * (1.0 / 4611686018427387904.0);
-    for (uint32_t e = tid; e < qq * CH; e += WM_THREADS) {
-        const uint32_t jj = e & 31, ac = e >> 5;
-        const uint32_t mm = c * CH + jj;
-        if (mm > i && mm < L)
-            out[(size_t)pair_index(i, mm, L) * qq + ac] = (float)((double)hist64[e] * back);
+    for (uint32_t e = tid; e < qq * WM_COLS; e += WM_THREADS) {
+        const uint32_t col = e & (WM_COLS - 1), ab = e / WM_COLS;
+        const uint32_t mm = cp * WM_COLS + col;
+        const uint32_t v = hist32[e];
+        if (mm > i && mm < L && v) atomicAdd(&acc[(size_t)pair_index(i, mm, L) * qq + ab], v);
     }
+}
+
+// Pass 3, in place: bins of 2^-31 sum(w) to float (NaN if the total was bad).
+__global__ void k_fixed_to_float(uint32_t* __restrict__ acc, uint64_t n,
+                                 const float* __restrict__ wsum,
+                                 const uint32_t* __restrict__ bad) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const double back = bad[0] ? NAN : (double)wsum[0] * (1.0 / 2147483648.0);
+    reinterpret_cast<float*>(acc)[idx] = (float)((double)acc[idx] * back);
 }
 
 // The other regularisers of the coupling step (mcmc.cl:904-1243), one wave

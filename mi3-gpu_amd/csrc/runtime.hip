@@ -86,11 +86,12 @@ struct mi3_ctx {
     std::map<std::string, Chunked> chunked;  // chunked layouts keyed by source buffer
     double* red = nullptr;                    // reduction partials
     float* red_out = nullptr;                 // 2 floats: internal reduction result
-    uint64_t* wq = nullptr;                   // fixed-point weights
+    uint32_t* wq = nullptr;                   // fixed-point weights
     size_t wq_cap = 0;
     uint32_t* wbad = nullptr;
     uint32_t* wm_units = nullptr;
     uint32_t n_units = 0;
+    uint32_t wm_split = 1;
     uint8_t* seq1 = nullptr;                  // one sequence (fill_seqs)
     // host staging
     std::vector<Pinned> pins;
@@ -274,11 +275,25 @@ int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t s
         hipEventCreate(&c->t0);
         hipEventRecord(c->t0, c->stream);
     }
-    // weighted-marginal work units: (row i, chunk c) with chunk holding some m > i
+    // weighted-marginal work units: (row i, group cp of 64 partner sites)
+    // with the group holding some m > i; each unit's sequence range is split
+    // in wm_split parts, chosen to fill the last wave of work-groups best
     std::vector<uint32_t> units;
+    const uint32_t ngrp = (L + WM_COLS - 1) / WM_COLS;
     for (uint32_t i = 0; i + 1 < L; i++)
-        for (uint32_t ch = (i + 1) / CH; ch < c->nchunk; ch++) units.push_back((i << 8) | ch);
+        for (uint32_t cp = (i + 1) / WM_COLS; cp < ngrp; cp++) units.push_back((i << 8) | cp);
     c->n_units = (uint32_t)units.size();
+    {
+        double best = 0.0;
+        for (uint32_t ns = 1; ns <= 4; ns++) {
+            const uint32_t wgs = c->n_units * ns;
+            const double t = (double)((wgs + c->ncu - 1) / c->ncu) / ns;
+            if (ns == 1 || t < best - 1e-9) {
+                best = t;
+                c->wm_split = ns;
+            }
+        }
+    }
     if (hipMalloc(&c->wm_units, units.size() * 4) != hipSuccess ||
         hipMemcpy(c->wm_units, units.data(), units.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc(&c->red, RED_BLOCKS * 2 * sizeof(double)) != hipSuccess ||
@@ -696,10 +711,10 @@ int mi3_weighted_marg(mi3_ctx* c, const char* seqbuf, const char* wbuf, uint32_t
     if (c->wq_cap < nseq) {
         hipFree(c->wq);
         c->wq = nullptr;
-        HIPCHK(c, hipMalloc(&c->wq, (size_t)nseq * sizeof(uint64_t) + 256));
+        HIPCHK(c, hipMalloc(&c->wq, (size_t)nseq * sizeof(uint32_t) + 256));
         c->wq_cap = nseq;
     }
-    size_t lds = qq * CH * 8;
+    size_t lds = qq * WM_COLS * 4;
     HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_weighted_marg),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     Scope s(c, "weightedMarg", (size_t)nseq * (c->L + 8));
@@ -712,10 +727,16 @@ int mi3_weighted_marg(mi3_ctx* c, const char* seqbuf, const char* wbuf, uint32_t
                        nb, 1, c->red_out);
     hipLaunchKernelGGL(k_weights_fixed, dim3(grid1(nseq ? nseq : 1)), dim3(256), 0, c->stream,
                        (const float*)wb->d, nseq, (const float*)c->red_out, c->wq, c->wbad);
-    hipLaunchKernelGGL(k_weighted_marg, dim3(c->n_units), dim3(WM_THREADS), lds, c->stream,
-                       (const uint8_t*)sb->d, (uint32_t)ld, nseq, (const uint64_t*)c->wq,
-                       (const float*)c->red_out, (const uint32_t*)c->wbad, c->L, c->q,
-                       (const uint32_t*)c->wm_units, (float*)bb->d);
+    const uint64_t nbins = (uint64_t)c->nP * qq;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid1(nbins)), dim3(256), 0, c->stream, (uint32_t*)bb->d,
+                       0u, nbins);
+    hipLaunchKernelGGL(k_weighted_marg, dim3(c->n_units * c->wm_split), dim3(WM_THREADS), lds,
+                       c->stream, (const uint8_t*)sb->d, (uint32_t)ld, nseq,
+                       (const uint32_t*)c->wq, c->L, c->q, (const uint32_t*)c->wm_units,
+                       c->wm_split, (uint32_t*)bb->d);
+    hipLaunchKernelGGL(k_fixed_to_float, dim3(grid1(nbins)), dim3(256), 0, c->stream,
+                       (uint32_t*)bb->d, nbins, (const float*)c->red_out,
+                       (const uint32_t*)c->wbad);
     HIPCHK(c, hipGetLastError());
     bb->version++;
     return MI3_OK;
