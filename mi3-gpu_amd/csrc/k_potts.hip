@@ -365,20 +365,24 @@ __device__ __forceinline__ void sweep_block(
         const float dE = transposed_sum<S>(acc);
         const float u = (float)(Pcur >> 8) * 0x1.0p-24f;
         const uint64_t amask = __ballot(__expf(nbeta * dE) > u);
-        const uint64_t own_m = (1ull << owner) | (1ull << (owner + 32));
+        // the owner lanes of pos take the proposals their half-wave accepted:
+        // all proposal broadcasts are issued together (one LDS wait), the
+        // state word holding pos is chosen once per step, and each slot's
+        // lane mask takes three scalar operations
+        uint32_t bnew[S];
 #pragma unroll
-        for (int s = 0; s < S; s++) {
-            const uint64_t mA = ((amask >> (s << SH)) & 1ull) ? 0x00000000FFFFFFFFull : 0ull;
-            const uint64_t mB = ((amask >> ((s << SH) + 32)) & 1ull) ? 0xFFFFFFFF00000000ull : 0ull;
-            const uint64_t m = (mA | mB) & own_m;
-            if (m) {
-                const uint32_t bsh = (uint32_t)half_bcast(bmine, s << SH) << shp;
+        for (int s = 0; s < S; s++) bnew[s] = (uint32_t)half_bcast(bmine, s << SH);
+        const uint32_t keep = ~(0xffu << shp);
 #pragma unroll
-                for (int kw = 0; kw < KW; kw++) {
-                    if (kwp == (uint32_t)kw) {
-                        uint32_t nw = (st[s][kw] & ~(0xffu << shp)) | bsh;
-                        st[s][kw] = lane_select_u(m, nw, st[s][kw]);
-                    }
+        for (int kw = 0; kw < KW; kw++) {
+            if (kwp == (uint32_t)kw) {
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    // bits 0 and 32 of the shifted ballot are the two
+                    // half-waves' decisions for slot s; moved to the owner lanes
+                    const uint64_t m = ((amask >> (s << SH)) & 0x0000000100000001ull) << owner;
+                    const uint32_t nw = (st[s][kw] & keep) | (bnew[s] << shp);
+                    st[s][kw] = lane_select_u(m, nw, st[s][kw]);
                 }
             }
         }
