@@ -198,6 +198,14 @@ __device__ __forceinline__ uint32_t byte_x128(uint32_t w, int e) {
     return r;
 }
 
+// a * b + c for 24-bit a, b in one op (the compiler turns the add of
+// disjoint bit ranges into an or, which then stays a separate instruction).
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 __device__ __forceinline__ uint32_t lane_select_u(uint64_t mask, uint32_t a, uint32_t b) {
     uint32_t r;
     asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(mask));
@@ -302,8 +310,11 @@ __device__ __forceinline__ void sweep_block(
             Po = ((r.w >> 8) << 8) | mulhi_range(r.z, q);
         }
         const uint32_t Pcur = (t & 1) ? Po : Pe;
-        const int bmine = (int)(Pcur & 0xffu);
+        const uint32_t bmine = Pcur & 0xffu;
         const int own_addr = (int)((owner + (lane & 32)) * 4);
+        // the proposal as a row offset, scaled by its sender so that every
+        // receiving lane adds only its column
+        const int brow = (int)__umul24(bmine, qch4);
         // residues a (from the owner lane of pos) and proposals b of every
         // slot, fetched in groups ahead of their use
         constexpr int GB = (S < 8) ? S : 8;
@@ -313,14 +324,14 @@ __device__ __forceinline__ void sweep_block(
 #pragma unroll
             for (int g = 0; g < GB; g++) {
                 const int s = s0 + g;
-                uint32_t av = (select_word_opaque<KW>(st[s], kwp) >> shp) & 0xffu;
+                const uint32_t av = __builtin_amdgcn_ubfe(select_word_opaque<KW>(st[s], kwp), shp, 8);
                 a[g] = (uint32_t)__builtin_amdgcn_ds_bpermute(own_addr, (int)av);
-                b[g] = (uint32_t)half_bcast(bmine, s << SH);
+                b[g] = (uint32_t)half_bcast(brow, s << SH);
             }
 #pragma unroll
             for (int g = 0; g < GB; g++) {
-                baseA[s0 + g] = __umul24(a[g], qch4) + j4;
-                baseB[s0 + g] = __umul24(b[g], qch4) + j4;
+                baseA[s0 + g] = mad_u24(a[g], qch4, j4);
+                baseB[s0 + g] = b[g] + j4;
                 acc[s0 + g] = 0.f;
             }
         }
@@ -370,8 +381,9 @@ __device__ __forceinline__ void sweep_block(
         // state word holding pos is chosen once per step, and each slot's
         // lane mask takes three scalar operations
         uint32_t bnew[S];
+        const int bsh = (int)(bmine << shp);  // the proposal at its byte of the word
 #pragma unroll
-        for (int s = 0; s < S; s++) bnew[s] = (uint32_t)half_bcast(bmine, s << SH);
+        for (int s = 0; s < S; s++) bnew[s] = (uint32_t)half_bcast(bsh, s << SH);
         const uint32_t keep = ~(0xffu << shp);
 #pragma unroll
         for (int kw = 0; kw < KW; kw++) {
@@ -381,7 +393,7 @@ __device__ __forceinline__ void sweep_block(
                     // bits 0 and 32 of the shifted ballot are the two
                     // half-waves' decisions for slot s; moved to the owner lanes
                     const uint64_t m = ((amask >> (s << SH)) & 0x0000000100000001ull) << owner;
-                    const uint32_t nw = (st[s][kw] & keep) | (bnew[s] << shp);
+                    const uint32_t nw = (st[s][kw] & keep) | bnew[s];
                     st[s][kw] = lane_select_u(m, nw, st[s][kw]);
                 }
             }
