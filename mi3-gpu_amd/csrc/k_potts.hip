@@ -52,16 +52,6 @@ __device__ __forceinline__ uint32_t state_word(const uint32_t (&w)[KW], uint32_t
         return select_word<KW>(w, k);
 }
 
-__device__ __forceinline__ float swz_xor_rt(float v, int m) {
-    switch (m) {
-        case 16: return swz_xor<16>(v);
-        case 8: return swz_xor<8>(v);
-        case 4: return swz_xor<4>(v);
-        case 2: return swz_xor<2>(v);
-        default: return swz_xor<1>(v);
-    }
-}
-
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
 
 // Wave-wide lane masks: bit `lane` set when (lane & m) != 0.
@@ -83,15 +73,50 @@ __device__ __forceinline__ float lane_select(uint64_t mask, float a, float b) {
     return r;
 }
 
+// Value of v in the partner lane of a reduction level below 16, within a
+// row of 16 lanes, by DPP: m=8 row_ror:8 (lane ^ 8), m=4 row_half_mirror
+// (lane ^ 7: the pair differs in bit 2, and in bits 0-1, which the later
+// levels sum over anyway), m=2 and m=1 quad permutations (lane ^ 2, ^ 1).
+__device__ __forceinline__ float dpp_partner(float v, int m) {
+    const int x = __float_as_int(v);
+    switch (m) {
+        case 8: return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false));
+        case 4: return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false));
+        case 2: return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false));
+        default: return __int_as_float(__builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false));
+    }
+}
+
 // Reduce S per-lane partial sums over the 32 lanes of each half-wave at once
 // (butterfly with operand exchange): afterwards lane j holds the full sum of
-// slot j >> (5 - log2 S).  Costs about 4*S + 5 operations instead of the
-// 10*S of S independent butterflies.
+// slot j >> (5 - log2 S).  The first level pairs lanes j and j + 16: one
+// v_permlane16_swap exchanges the odd 16-lane rows of v[s] with the even
+// rows of v[s + h], so one add leaves rows 0 and 2 with slot s and rows 1
+// and 3 with slot s + h, with no selects.  Each later level keeps one of two
+// slots by the lane's bit m (v_cndmask with constant lane masks) and adds
+// the other slot's value from the partner lane through DPP.  No LDS.
 template <int S>
 __device__ __forceinline__ float transposed_sum(float (&v)[S]) {
     int n = S;
+    if (n > 1) {
+        const int h = n / 2;
 #pragma unroll
-    for (int l = 0; l < 5; l++) {
+        for (int s = 0; s < S / 2; s++) {
+            if (s < h) {
+                const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[s]),
+                                                                __float_as_uint(v[s + h]), false,
+                                                                false);
+                v[s] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+            }
+        }
+        n = h;
+    } else {
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[0]),
+                                                        __float_as_uint(v[0]), false, false);
+        v[0] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+    }
+#pragma unroll
+    for (int l = 1; l < 5; l++) {
         const int m = 16 >> l;
         if (n > 1) {
             const int h = n / 2;
@@ -101,12 +126,12 @@ __device__ __forceinline__ float transposed_sum(float (&v)[S]) {
                 if (s < h) {
                     float send = lane_select(mk, v[s], v[s + h]);
                     float keep = lane_select(mk, v[s + h], v[s]);
-                    v[s] = keep + swz_xor_rt(send, m);
+                    v[s] = keep + dpp_partner(send, m);
                 }
             }
             n = h;
         } else {
-            v[0] += swz_xor_rt(v[0], m);
+            v[0] += dpp_partner(v[0], m);
         }
     }
     return v[0];
