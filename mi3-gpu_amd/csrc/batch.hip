@@ -111,7 +111,7 @@ void drop_graph(mi3_batch* b) {
 
 // Walkers per tile allowed for a class.
 std::vector<uint32_t> tile_options(uint32_t kw) {
-    if (kw >= 3) return {128, 256};
+    if (kw >= 4) return {128, 256};
     return {128, 256, 512};
 }
 
@@ -483,7 +483,7 @@ int mi3_batch_set_tile(mi3_batch* b, uint32_t model, uint32_t wpb) {
     ClassKey k{state_words(b->L[model]), pf_class(b->q[model])};
     auto opts = tile_options(k.kw);
     if (std::find(opts.begin(), opts.end(), wpb) == opts.end())
-        return bfail(b, MI3_EINVAL, "walkers per tile must be 128, 256 or 512 (at most 256 for L > 256)");
+        return bfail(b, MI3_EINVAL, "walkers per tile must be 128, 256 or 512 (at most 256 for L > 384)");
     hipSetDevice(b->dev);
     BCHK(b, hipStreamSynchronize(b->stream));
     drop_graph(b);

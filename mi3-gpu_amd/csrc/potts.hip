@@ -75,9 +75,11 @@ struct SweepDisp {
             if (w <= 256) return sweep_kws<KW, PF, 16>(a, st);
             return sweep_kws<KW, PF, 32>(a, st);
         } else {
-            // more state words per walker: at most 16 slots fit in registers
+            // more state words per walker: at most 16 slots fit in registers;
+            // with three words, 16 waves of 128 registers hold 512 walkers
             uint32_t w = a.wpb ? a.wpb : SWEEP_DEFAULT_WPB;
             if (w <= 128) return sweep_kws<KW, PF, 8>(a, st);
+            if (KW == 3 && w > 256) return sweep_kws<KW, PF, 16, 16>(a, st);
             return sweep_kws<KW, PF, 16>(a, st);
         }
     }
@@ -95,14 +97,14 @@ hipError_t launch_sweep(const SweepArgs& a, hipStream_t st) {
 }
 
 namespace {
-template <int KW, int PF, int S>
+template <int KW, int PF, int S, int NW = NWAVES>
 hipError_t batch_kws(const BatchModel* models, const BatchTile* tiles, uint32_t ntiles,
                      uint32_t* counter, uint32_t grid, hipStream_t st) {
-    auto kern = k_sweep_batch<KW, S, NWAVES, PF>;
+    auto kern = k_sweep_batch<KW, S, NW, PF>;
     size_t lds = 2ull * PF * POTTS_THREADS * 4 * sizeof(float);  // two padded chunks
     hipError_t e = set_lds(kern, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, models, tiles, ntiles, counter);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * NW), lds, st, models, tiles, ntiles, counter);
     return hipGetLastError();
 }
 
@@ -117,6 +119,8 @@ hipError_t batch_kw(uint32_t wpb, const BatchModel* models, const BatchTile* til
     } else {
         if (wpb == 128) return batch_kws<KW, PF, 8>(models, tiles, ntiles, counter, grid, st);
         if (wpb == 256) return batch_kws<KW, PF, 16>(models, tiles, ntiles, counter, grid, st);
+        if constexpr (KW == 3)
+            if (wpb == 512) return batch_kws<KW, PF, 16, 16>(models, tiles, ntiles, counter, grid, st);
         return hipErrorInvalidValue;
     }
 }

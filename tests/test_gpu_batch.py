@@ -59,7 +59,7 @@ def test_batch_chains_match_single_model_oracle(gpu_ok):
     b.close()
 
 
-@pytest.mark.parametrize("wpb", [128, 256])
+@pytest.mark.parametrize("wpb", [128, 256, 512])
 def test_batch_tiles_give_identical_chains(gpu_ok, wpb):
     b1, Js, S0 = _setup()
     b2, _, _ = _setup()
@@ -78,14 +78,18 @@ def test_batch_tune_and_errors(gpu_ok):
     b, Js, S0 = _setup()
     b.tune(1)
     for m, (L, q, N) in enumerate(MODELS):
-        assert b.tile(m) in ((128, 256) if L > 256 else (128, 256, 512))
+        assert b.tile(m) in ((128, 256) if L > 384 else (128, 256, 512))
         assert b.step_count(m) > 0
     b.run(1)
     b.wait()
     with pytest.raises(ValueError):
-        b.set_tile(4, 512)            # L = 260 allows at most 256
+        b.set_tile(4, 1024)           # 128, 256 or 512 walkers per tile
     with pytest.raises(ValueError):
         b.set_seqs(0, np.full((512, 20), 9, 'u1'))
     with pytest.raises(ValueError):
         PottsBatch([10], [4], [100])
     b.close()
+    long = PottsBatch([400], [4], [512])
+    with pytest.raises(ValueError):
+        long.set_tile(0, 512)         # L > 384 allows at most 256
+    long.close()
