@@ -499,7 +499,10 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
     extern __shared__ f4v lds4[];
     float* lds = reinterpret_cast<float*>(lds4);
 
-    constexpr uint32_t chf = PF * NT * 4;  // padded chunk (floats)
+    // the padded chunk is PF float4 per thread of the first DMA_T threads
+    // (the chunk layout's unit); waves beyond them take no part in staging
+    constexpr int DMA_T = NT < 512 ? NT : 512;
+    constexpr uint32_t chf = PF * DMA_T * 4;  // padded chunk (floats)
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const uint32_t j = lane & 31;
@@ -534,7 +537,7 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
     char* ldsb = reinterpret_cast<char*>(lds4);
     if (L >= 2) {
         // first unit: row 0, chunk 0
-        chunk_dma<PF, NT>(rJ, voff, 0, ldsb, wave);
+        if (wave < DMA_T / 64) chunk_dma<PF, DMA_T>(rJ, voff, 0, ldsb, wave);
         wait_vmem();
         __syncthreads();
         uint32_t buf = 0;
@@ -558,8 +561,9 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
                     if (has_next) {
                         uint32_t ni = last_chunk ? i + 1 : i;
                         uint32_t nc = last_chunk ? ((i + 2) >> 5) : (uint32_t)k + 1;
-                        chunk_dma<PF, NT>(rJ, voff, (ni * nchunk + nc) * chf * 4,
-                                          ldsb + (buf ^ 1) * chf * 4, wave);
+                        if (wave < DMA_T / 64)
+                            chunk_dma<PF, DMA_T>(rJ, voff, (ni * nchunk + nc) * chf * 4,
+                                                 ldsb + (buf ^ 1) * chf * 4, wave);
                     }
                     const float* cb = lds + buf * chf;
                     if constexpr (S > 16) {

@@ -7,14 +7,19 @@ namespace mi3 {
 namespace {
 constexpr int NWAVES = 8;
 constexpr int NT = 64 * NWAVES;
-// walker slots per half-wave: fewer for long sequences so state stays in registers
-template <int KW>
-constexpr int slots() { return KW <= 2 ? 16 : 8; }
+
+// Launch shapes (walker slots per half-wave S, waves NW): W = 2*S*NW walkers
+// per work-group.  16 waves of at most 128 registers (4 waves per SIMD) hide
+// the gathers' latency better than 8 waves of 256, wherever the state fits:
+// measured on MI355X at L=120/232/300/450, q=21 and q=4.
 // sequence slots per half-wave of the energy kernel: every staged coupling
 // chunk is read once per sequence, so more sequences per work-group cut the
-// L2 -> LDS traffic per sequence
+// L2 -> LDS traffic per sequence; 32 slots in 8 waves for 2 state words,
+// 16 slots in 16 waves otherwise (measured the faster)
 template <int KW>
-constexpr int eslots() { return KW <= 2 ? 32 : 16; }
+constexpr int eslots() { return KW == 2 ? 32 : 16; }
+template <int KW>
+constexpr int ewaves() { return KW == 2 || KW == 4 ? 8 : 16; }
 
 static_assert(NT == POTTS_THREADS, "thread count");
 
@@ -41,14 +46,15 @@ hipError_t sweep_kws(const SweepArgs& a, hipStream_t st) {
 
 template <int KW, int PF>
 hipError_t energies_kw(const EnergyArgs& a, hipStream_t st) {
-    constexpr uint32_t W = 2 * eslots<KW>() * NWAVES;
-    auto kern = k_energies<KW, eslots<KW>(), NWAVES, PF>;
+    constexpr int NW = ewaves<KW>();
+    constexpr uint32_t W = 2 * eslots<KW>() * NW;
+    auto kern = k_energies<KW, eslots<KW>(), NW, PF>;
     size_t lds = 2ull * chunk_floats(a.q) * sizeof(float);
     hipError_t e = set_lds(kern, lds);
     if (e != hipSuccess) return e;
     dim3 grid((a.nseq + W - 1) / W);
-    hipLaunchKernelGGL(kern, grid, dim3(NT), lds, st, a.Ju, a.seq, a.E, a.ld, a.nseq, a.L, a.q,
-                       a.nchunk);
+    hipLaunchKernelGGL(kern, grid, dim3(64 * NW), lds, st, a.Ju, a.seq, a.E, a.ld, a.nseq, a.L,
+                       a.q, a.nchunk);
     return hipGetLastError();
 }
 
@@ -69,18 +75,14 @@ hipError_t dispatch(const A& a, hipStream_t st, uint32_t nchunk, uint32_t q) {
 template <int KW, int PF>
 struct SweepDisp {
     static hipError_t run(const SweepArgs& a, hipStream_t st) {
-        if constexpr (KW <= 2) {
-            uint32_t w = a.wpb ? a.wpb : SWEEP_DEFAULT_WPB;
-            if (w <= 128) return sweep_kws<KW, PF, 8>(a, st);
-            if (w <= 256) return sweep_kws<KW, PF, 16>(a, st);
-            return sweep_kws<KW, PF, 32>(a, st);
+        uint32_t w = a.wpb ? a.wpb : SWEEP_DEFAULT_WPB;
+        if (w <= 128) return sweep_kws<KW, PF, 8, 8>(a, st);
+        if constexpr (KW <= 3) {
+            if (w <= 256) return sweep_kws<KW, PF, 8, 16>(a, st);
+            return sweep_kws<KW, PF, 16, 16>(a, st);
         } else {
-            // more state words per walker: at most 16 slots fit in registers;
-            // with three words, 16 waves of 128 registers hold 512 walkers
-            uint32_t w = a.wpb ? a.wpb : SWEEP_DEFAULT_WPB;
-            if (w <= 128) return sweep_kws<KW, PF, 8>(a, st);
-            if (KW == 3 && w > 256) return sweep_kws<KW, PF, 16, 16>(a, st);
-            return sweep_kws<KW, PF, 16>(a, st);
+            // four state words: 256 walkers at most
+            return sweep_kws<KW, PF, 8, 16>(a, st);
         }
     }
 };
@@ -111,18 +113,11 @@ hipError_t batch_kws(const BatchModel* models, const BatchTile* tiles, uint32_t 
 template <int KW, int PF>
 hipError_t batch_kw(uint32_t wpb, const BatchModel* models, const BatchTile* tiles,
                     uint32_t ntiles, uint32_t* counter, uint32_t grid, hipStream_t st) {
-    if constexpr (KW <= 2) {
-        if (wpb == 128) return batch_kws<KW, PF, 8>(models, tiles, ntiles, counter, grid, st);
-        if (wpb == 256) return batch_kws<KW, PF, 16>(models, tiles, ntiles, counter, grid, st);
-        if (wpb == 512) return batch_kws<KW, PF, 32>(models, tiles, ntiles, counter, grid, st);
-        return hipErrorInvalidValue;
-    } else {
-        if (wpb == 128) return batch_kws<KW, PF, 8>(models, tiles, ntiles, counter, grid, st);
-        if (wpb == 256) return batch_kws<KW, PF, 16>(models, tiles, ntiles, counter, grid, st);
-        if constexpr (KW == 3)
-            if (wpb == 512) return batch_kws<KW, PF, 16, 16>(models, tiles, ntiles, counter, grid, st);
-        return hipErrorInvalidValue;
-    }
+    if (wpb == 128) return batch_kws<KW, PF, 8, 8>(models, tiles, ntiles, counter, grid, st);
+    if (wpb == 256) return batch_kws<KW, PF, 8, 16>(models, tiles, ntiles, counter, grid, st);
+    if constexpr (KW <= 3)
+        if (wpb == 512) return batch_kws<KW, PF, 16, 16>(models, tiles, ntiles, counter, grid, st);
+    return hipErrorInvalidValue;
 }
 }  // namespace
 
