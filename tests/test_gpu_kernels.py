@@ -26,7 +26,7 @@ def make_gpu(tmp_path, L, q, N, seed=1234, nsteps=None, profile=False, wgsize=No
     return g
 
 
-SHAPES = [(99, 4), (232, 21), (37, 7), (5, 3), (300, 21), (64, 21), (33, 2)]
+SHAPES = [(99, 4), (232, 21), (37, 7), (5, 3), (300, 21), (64, 21), (33, 2), (450, 4)]
 
 
 @pytest.mark.parametrize("L,q", SHAPES)
@@ -207,8 +207,8 @@ def test_update_reg_matches_oracle(gpu_ok, tmp_path, reg, fused):
     assert amb.mean() < 0.01
 
 
-def test_weighted_marg_and_renorm(gpu_ok, tmp_path):
-    L, q, N = 70, 21, 1024
+@pytest.mark.parametrize("L,q,N", [(70, 21, 1024), (130, 4, 1536)])
+def test_weighted_marg_and_renorm(gpu_ok, tmp_path, L, q, N):
     g, J, s, bt, bi, dJ = _newton_setup(tmp_path, L, q, N)
     w = np.random.default_rng(1).exponential(size=N).astype('<f4')
     g.setBuf('weights', w)
@@ -218,10 +218,31 @@ def test_weighted_marg_and_renorm(gpu_ok, tmp_path):
     np.testing.assert_allclose(F, Fx, rtol=1e-5, atol=1e-6 * w.sum())
     g.renormalize_bimarg()
     np.testing.assert_allclose(g.getBuf('bi').read(), po.renormalize(Fx), rtol=1e-5, atol=1e-7)
-    # unit weights reduce to the counts
+    # unit weights reduce to the counts (exactly when N is a power of two: each
+    # weight is then exactly 2^31 / N in the fixed-point form)
     g.setBuf('weights', np.ones(N, '<f4'))
     g.weightedMarg('main')
-    np.testing.assert_array_equal(g.getBuf('bi').read(), cboracle.bicounts(s, q).astype('f4'))
+    counts = cboracle.bicounts(s, q).astype('f4')
+    if N & (N - 1) == 0:
+        np.testing.assert_array_equal(g.getBuf('bi').read(), counts)
+    else:
+        np.testing.assert_allclose(g.getBuf('bi').read(), counts, rtol=1e-6)
+
+
+def test_weighted_marg_ragged_large(gpu_ok, tmp_path):
+    """A sequence count that is not a multiple of the kernel's groups (the
+    tail path), through the large buffers."""
+    L, q, N, n = 70, 21, 512, 1000
+    g, J, s0, bt, bi, dJ = _newton_setup(tmp_path, L, q, N)
+    g.initLargeBufs(n)
+    s = seqs_rand(n, L, q, 9)
+    g.setBuf('seq large', s)
+    w = np.random.default_rng(3).exponential(size=n).astype('<f4')
+    g.setBuf('weights large', w)
+    g.weightedMarg('large')
+    F = g.getBuf('bi').read()
+    Fx = cboracle.weighted_marg(s, w, q)
+    np.testing.assert_allclose(F, Fx, rtol=1e-5, atol=1e-6 * w.sum())
 
 
 def test_min_weights_stats(gpu_ok, tmp_path):
