@@ -137,7 +137,7 @@ int mi3_copy_subseq(mi3_ctx *ctx, const char *src, uint32_t seqind, const char *
  * powers of two are rounded into that range).  Takes the role of the
  * reference's work-group size option (WGSIZE compile define,
  * mcmcGPU.py:1080-1093; wgsize_heuristic mcmcGPU.py:1121-1132; --wgsize
- * Mi3.py:115).  Sequences longer than 384 use at most 256. */
+ * Mi3.py:115).  A request of 512 runs tiles of 256 when L > 384 and q <= 8. */
 int mi3_set_wgsize(mi3_ctx *ctx, uint32_t wgsize);
 
 /* Set nsteps per runMCMC call and the walker-stream offset (initMCMC,
@@ -276,7 +276,7 @@ int mi3_batch_run(mi3_batch *b, uint32_t nsweeps);
 int mi3_batch_synchronize(mi3_batch *b);
 
 /* Autotuner: for every class, time each allowed walkers-per-tile choice
- * (128, 256, 512; at most 256 for L > 384) on nsweeps sweeps of its models and
+ * (128, 256, 512) on nsweeps sweeps of its models and
  * keep the fastest.  The timed sweeps are real MCMC steps of the walkers. */
 int mi3_batch_tune(mi3_batch *b, uint32_t nsweeps);
 int mi3_batch_tile(mi3_batch *b, uint32_t model, uint32_t *wpb);

@@ -110,10 +110,7 @@ void drop_graph(mi3_batch* b) {
 }
 
 // Walkers per tile allowed for a class.
-std::vector<uint32_t> tile_options(uint32_t kw) {
-    if (kw >= 4) return {128, 256};
-    return {128, 256, 512};
-}
+std::vector<uint32_t> tile_options(uint32_t) { return {128, 256, 512}; }
 
 // Tiles of a class for a walkers-per-tile choice; larger models first so
 // that the tail of the persistent loop is short.
@@ -483,7 +480,7 @@ int mi3_batch_set_tile(mi3_batch* b, uint32_t model, uint32_t wpb) {
     ClassKey k{state_words(b->L[model]), pf_class(b->q[model])};
     auto opts = tile_options(k.kw);
     if (std::find(opts.begin(), opts.end(), wpb) == opts.end())
-        return bfail(b, MI3_EINVAL, "walkers per tile must be 128, 256 or 512 (at most 256 for L > 384)");
+        return bfail(b, MI3_EINVAL, "walkers per tile must be 128, 256 or 512");
     hipSetDevice(b->dev);
     BCHK(b, hipStreamSynchronize(b->stream));
     drop_graph(b);

@@ -22,8 +22,9 @@ struct EnergyArgs {
     uint32_t ld, nseq, L, q, nchunk;
 };
 
-// Walkers per work-group of the sweep kernel: 128, 256 or 512 for L <= 384,
-// 128 or 256 for longer sequences; the walker count must be a multiple of 512.
+// Walkers per work-group of the sweep kernel: 128, 256 or 512 (a request of
+// 512 runs tiles of 256 for L > 384 and q <= 8); the walker count must be a
+// multiple of 512.
 constexpr uint32_t SWEEP_WALKERS_PER_BLOCK = 512;
 constexpr uint32_t SWEEP_DEFAULT_WPB = 512;
 
@@ -72,9 +73,8 @@ hipError_t launch_energies(const EnergyArgs& a, hipStream_t st);
 inline uint32_t state_words(uint32_t L) { return (L + 127) / 128; }
 
 // Persistent sweep over `ntiles` tiles of models that share state_words(L)
-// == kw and pf_class(q) == pf, with `wpb` walkers per tile (128, 256 or 512;
-// at most 256 for kw >= 4), on `grid` work-groups.  `counter` (one u32) must
-// be zero at launch.
+// == kw and pf_class(q) == pf, with `wpb` walkers per tile (128, 256 or 512),
+// on `grid` work-groups.  `counter` (one u32) must be zero at launch.
 hipError_t launch_sweep_batch(uint32_t kw, uint32_t pf, uint32_t wpb, const BatchModel* models,
                               const BatchTile* tiles, uint32_t ntiles, uint32_t* counter,
                               uint32_t grid, hipStream_t st);

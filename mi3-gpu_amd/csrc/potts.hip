@@ -19,7 +19,7 @@ constexpr int NT = 64 * NWAVES;
 template <int KW>
 constexpr int eslots() { return KW == 2 ? 32 : 16; }
 template <int KW>
-constexpr int ewaves() { return KW == 2 || KW == 4 ? 8 : 16; }
+constexpr int ewaves() { return KW == 2 ? 8 : 16; }
 
 static_assert(NT == POTTS_THREADS, "thread count");
 
@@ -77,13 +77,13 @@ struct SweepDisp {
     static hipError_t run(const SweepArgs& a, hipStream_t st) {
         uint32_t w = a.wpb ? a.wpb : SWEEP_DEFAULT_WPB;
         if (w <= 128) return sweep_kws<KW, PF, 8, 8>(a, st);
-        if constexpr (KW <= 3) {
-            if (w <= 256) return sweep_kws<KW, PF, 8, 16>(a, st);
-            return sweep_kws<KW, PF, 16, 16>(a, st);
-        } else {
-            // four state words: 256 walkers at most
+        if (w <= 256) return sweep_kws<KW, PF, 8, 16>(a, st);
+        // four state words do not fit 16 slots in 128 registers; the spills
+        // still pay for q > 8, not for the small chunks of q <= 8
+        if constexpr (KW == 4 && PF == 1)
             return sweep_kws<KW, PF, 8, 16>(a, st);
-        }
+        else
+            return sweep_kws<KW, PF, 16, 16>(a, st);
     }
 };
 template <int KW, int PF>
@@ -115,8 +115,7 @@ hipError_t batch_kw(uint32_t wpb, const BatchModel* models, const BatchTile* til
                     uint32_t ntiles, uint32_t* counter, uint32_t grid, hipStream_t st) {
     if (wpb == 128) return batch_kws<KW, PF, 8, 8>(models, tiles, ntiles, counter, grid, st);
     if (wpb == 256) return batch_kws<KW, PF, 8, 16>(models, tiles, ntiles, counter, grid, st);
-    if constexpr (KW <= 3)
-        if (wpb == 512) return batch_kws<KW, PF, 16, 16>(models, tiles, ntiles, counter, grid, st);
+    if (wpb == 512) return batch_kws<KW, PF, 16, 16>(models, tiles, ntiles, counter, grid, st);
     return hipErrorInvalidValue;
 }
 }  // namespace

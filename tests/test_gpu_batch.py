@@ -78,7 +78,7 @@ def test_batch_tune_and_errors(gpu_ok):
     b, Js, S0 = _setup()
     b.tune(1)
     for m, (L, q, N) in enumerate(MODELS):
-        assert b.tile(m) in ((128, 256) if L > 384 else (128, 256, 512))
+        assert b.tile(m) in (128, 256, 512)
         assert b.step_count(m) > 0
     b.run(1)
     b.wait()
@@ -87,9 +87,7 @@ def test_batch_tune_and_errors(gpu_ok):
     with pytest.raises(ValueError):
         b.set_seqs(0, np.full((512, 20), 9, 'u1'))
     with pytest.raises(ValueError):
+        b.set_tile(0, 64)
+    with pytest.raises(ValueError):
         PottsBatch([10], [4], [100])
     b.close()
-    long = PottsBatch([400], [4], [512])
-    with pytest.raises(ValueError):
-        long.set_tile(0, 512)         # L > 384 allows at most 256
-    long.close()

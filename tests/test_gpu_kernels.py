@@ -104,7 +104,8 @@ def test_bicounts_ragged_large_buffer(gpu_ok, tmp_path):
                                              (8, 3, 512, 81, 512), (256, 5, 512, 40, 512),
                                              (300, 21, 1024, 30, 256), (400, 4, 512, 20, 256),
                                              (500, 4, 512, 12, 128), (350, 21, 1024, 20, 512),
-                                             (384, 4, 512, 16, 512)])
+                                             (384, 4, 512, 16, 512), (450, 21, 512, 12, 512),
+                                             (420, 5, 512, 12, 512)])
 def test_sweep_matches_oracle_trajectories(gpu_ok, tmp_path, L, q, N, nsteps, wg):
     J = model(L, q, 7, sj=0.3 if q < 10 else 0.1)
     s0 = seqs_rand(N, L, q, 8)
