@@ -21,7 +21,9 @@ Printed as one JSON line, with
   roofline:     algorithmic bytes (L+1 per site update: read the walker's L
                 residues, write back one) per launch over the launch duration
                 measured with HIP events on the engine's stream, against the
-                8 TB/s HBM peak;
+                8 TB/s HBM peak, and against the best bandwidth plain
+                streaming (copies, a read-only stream) reaches in the same run
+                (achievable);
   cpu_baseline: the CPU port of the same sampler (oracle/oracle.c, float32
                 as the reference CPU sampler) on this host's cores, on a
                 bounded sample of the same workload.
@@ -441,6 +443,38 @@ def copy_calibration(mib):
     lib().mi3_destroy(ctx)
 
 
+def achievable_hbm(mib=4096, reps=10):
+    """HBM bandwidth a plain copy reaches (read + write bytes over wall time
+    between synchronisations), GB/s: the better of the runtime's
+    device-to-device copy and the library's streaming-copy kernel."""
+    from mi3gpu._lib import lib, check
+    import ctypes
+    ctx = ctypes.c_void_p()
+    check(None, lib().mi3_create(0, 8, 4, 512, 1, 0, ctypes.byref(ctx)))
+    nbytes = mib << 20
+    for name in (b'copy src', b'copy dst'):
+        check(ctx, lib().mi3_buf_alloc(ctx, name, nbytes))
+    check(ctx, lib().mi3_buf_fill_u32(ctx, b'copy src', 0x01020304))
+    rates = {}
+    runs = (('hipMemcpy device-to-device copy', 2,
+             lambda: lib().mi3_buf_copy(ctx, b'copy dst', b'copy src', nbytes)),
+            ('streaming-copy kernel', 2,
+             lambda: lib().mi3_stream_copy(ctx, b'copy dst', b'copy src', nbytes)),
+            ('streaming-read kernel', 1,
+             lambda: lib().mi3_stream_read(ctx, b'copy src', nbytes, b'copy dst')))
+    for kind, passes, fn in runs:
+        check(ctx, fn())
+        check(ctx, lib().mi3_synchronize(ctx))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            check(ctx, fn())
+        check(ctx, lib().mi3_synchronize(ctx))
+        rates[kind] = passes * nbytes * reps / (time.perf_counter() - t0) / 1e9
+    lib().mi3_destroy(ctx)
+    kind = max(rates, key=rates.get)
+    return rates[kind], f"{kind} of {mib} MiB x {reps}, bytes moved over wall time", rates
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -503,6 +537,7 @@ def main():
 
     if a.copy_calibration > 0:
         copy_calibration(a.copy_calibration)
+    hbm_copy, hbm_kind, hbm_rates = achievable_hbm()
 
     su_per_step = N * nsteps
     value = su_per_step * a.steps / dt
@@ -531,7 +566,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(
                          (L, q, N, a.sweeps_per_step, a.wgsize)),
                      "kernel": "k_sweep", "kernel_ms": kms, "launches": kcnt,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     "achievable": hbm_copy, "frac_of_achievable": achieved / hbm_copy,
+                     "achievable_kind": hbm_kind,
+                     "achievable_all": hbm_rates},
     }
     if a.ess_sweeps > 0 or a.rounds > 0 or a.made:
         tseqs, target = sample_target(J, L, q, 2 ** 20, a.target_sweeps, 4, a.wgsize)

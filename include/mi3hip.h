@@ -88,6 +88,17 @@ int mi3_buf_write(mi3_ctx *ctx, const char *name, const void *src, size_t nbytes
 /* Device-to-device copy (setBuf with a device buffer, mcmcGPU.py:873-879). */
 int mi3_buf_copy(mi3_ctx *ctx, const char *dst, const char *src, size_t nbytes);
 
+/* Copy `nbytes` (a multiple of 16) between two buffers with a plain
+ * streaming kernel: the reference for the HBM bandwidth a kernel reaches
+ * (bench.py reports the sweep against it).  No counterpart in the reference. */
+int mi3_stream_copy(mi3_ctx *ctx, const char *dst, const char *src, size_t nbytes);
+
+/* Read `nbytes` (a multiple of 16) of a buffer with a plain streaming kernel
+ * (the read-only bandwidth reference); `out` (4 bytes) is written only in the
+ * unlikely case the data's xor equals an internal key.  No counterpart in
+ * the reference. */
+int mi3_stream_read(mi3_ctx *ctx, const char *src, size_t nbytes, const char *out);
+
 /* Enqueue a read of `nbytes` at `offset` into library-owned pinned staging;
  * returns a ticket.  The data is a snapshot taken in stream order, as
  * getBuf's enqueue_copy (mcmcGPU.py:845-867). */

@@ -32,6 +32,46 @@ __global__ void k_unpack_chunked(const float* __restrict__ J, float* __restrict_
     Ju[idx] = v;
 }
 
+// Plain streaming kernels over n 16-byte words (grid-stride, eight
+// nontemporal loads in flight per thread): the references for the HBM
+// bandwidth a kernel reaches.  The read kernel stores the xor of the data
+// only if it equals a key the caller passes, so that the loads stay.
+typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+constexpr int STREAM_U = 8;
+
+__global__ __launch_bounds__(256) void k_stream_copy(const u4v* __restrict__ src,
+                                                     u4v* __restrict__ dst, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (STREAM_U - 1) * stride < n; i += STREAM_U * stride) {
+        u4v v[STREAM_U];
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) v[u] = __builtin_nontemporal_load(&src[i + u * stride]);
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) __builtin_nontemporal_store(v[u], &dst[i + u * stride]);
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(256) void k_stream_read(const u4v* __restrict__ src, uint64_t n,
+                                                     uint32_t key, uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t acc = 0;
+    for (; i + (STREAM_U - 1) * stride < n; i += STREAM_U * stride) {
+        u4v v[STREAM_U];
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) v[u] = __builtin_nontemporal_load(&src[i + u * stride]);
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) acc ^= v[u].x ^ v[u].y ^ v[u].z ^ v[u].w;
+    }
+    for (; i < n; i += stride) {
+        const u4v a = src[i];
+        acc ^= a.x ^ a.y ^ a.z ^ a.w;
+    }
+    if (acc == key) out[0] = acc;
+}
+
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t val, uint64_t n) {
     uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < n) p[idx] = val;

@@ -407,6 +407,38 @@ int mi3_buf_copy(mi3_ctx* c, const char* dst, const char* src, size_t nbytes) {
     return MI3_OK;
 }
 
+int mi3_stream_copy(mi3_ctx* c, const char* dst, const char* src, size_t nbytes) {
+    if (!c) return MI3_EINVAL;
+    if (nbytes % 16) return fail(c, MI3_EINVAL, "stream copy: nbytes must be a multiple of 16");
+    Buf *bd, *bs;
+    int rc = need(c, dst, nbytes, &bd);
+    if (rc) return rc;
+    rc = need(c, src, nbytes, &bs);
+    if (rc) return rc;
+    Scope s(c, "streamCopy", 2 * nbytes);
+    hipLaunchKernelGGL(k_stream_copy, dim3(c->ncu * 256), dim3(256), 0, c->stream,
+                       (const u4v*)bs->d, (u4v*)bd->d, (uint64_t)(nbytes / 16));
+    HIPCHK(c, hipGetLastError());
+    bd->version++;
+    return MI3_OK;
+}
+
+int mi3_stream_read(mi3_ctx* c, const char* src, size_t nbytes, const char* out) {
+    if (!c) return MI3_EINVAL;
+    if (nbytes % 16) return fail(c, MI3_EINVAL, "stream read: nbytes must be a multiple of 16");
+    Buf *bs, *bo;
+    int rc = need(c, src, nbytes, &bs);
+    if (rc) return rc;
+    rc = need(c, out, 4, &bo);
+    if (rc) return rc;
+    Scope s(c, "streamRead", nbytes);
+    hipLaunchKernelGGL(k_stream_read, dim3(c->ncu * 256), dim3(256), 0, c->stream,
+                       (const u4v*)bs->d, (uint64_t)(nbytes / 16), 0x5A5A5A5Au,
+                       (uint32_t*)bo->d);
+    HIPCHK(c, hipGetLastError());
+    return MI3_OK;
+}
+
 int mi3_buf_read_async(mi3_ctx* c, const char* name, size_t nbytes, size_t offset, int* ticket) {
     if (!c || !ticket) return MI3_EINVAL;
     Buf* b;

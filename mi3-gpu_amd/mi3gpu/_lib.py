@@ -35,6 +35,8 @@ SIGNATURES = {
     'mi3_buf_size': (_i, [_P, _S, _c.POINTER(_sz)]),
     'mi3_buf_write': (_i, [_P, _S, _P, _sz, _sz]),
     'mi3_buf_copy': (_i, [_P, _S, _S, _sz]),
+    'mi3_stream_copy': (_i, [_P, _S, _S, _sz]),
+    'mi3_stream_read': (_i, [_P, _S, _sz, _S]),
     'mi3_buf_read_async': (_i, [_P, _S, _sz, _sz, _c.POINTER(_i)]),
     'mi3_ticket_wait': (_i, [_P, _i, _P, _sz]),
     'mi3_buf_fill_u32': (_i, [_P, _S, _u32]),
