@@ -55,7 +55,7 @@ int mi3_device_info(int device, char *buf, size_t buflen);
  * walkers (multiple of 512) on `device`.  Replaces MCMCGPU.__init__
  * (mcmcGPU.py:111-193) together with setup_GPU_context / initGPU
  * (mcmcGPU.py:1038-1119): no program is compiled at run time, L and q are
- * kernel arguments (2 <= L <= 512, 2 <= q <= 25: walker residues are held in
+ * kernel arguments (2 <= L <= 1024, 2 <= q <= 25: walker residues are held in
  * registers; MI3_EUNSUP beyond).  `seed` keys every counter-based random
  * stream.  `profile` != 0 records one event pair per launch (the reference's
  * PROFILING_ENABLE queue, mcmcGPU.py:153-157). */

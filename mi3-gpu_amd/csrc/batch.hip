@@ -110,7 +110,10 @@ void drop_graph(mi3_batch* b) {
 }
 
 // Walkers per tile allowed for a class.
-std::vector<uint32_t> tile_options(uint32_t) { return {128, 256, 512}; }
+std::vector<uint32_t> tile_options(uint32_t kw) {
+    if (kw >= 5) return {128, 256};
+    return {128, 256, 512};
+}
 
 // Tiles of a class for a walkers-per-tile choice; larger models first so
 // that the tail of the persistent loop is short.
@@ -232,10 +235,11 @@ int mi3_batch_create(int device, uint32_t nmodels, const uint32_t* L, const uint
         return MI3_EINVAL;
     }
     for (uint32_t m = 0; m < nmodels; m++) {
-        if (L[m] < 2 || q[m] < 2 || q[m] > MAX_Q || L[m] > MAX_L) {
+        if (L[m] < 2 || q[m] < 2 || q[m] > MAX_Q || L[m] > MAX_L || !chunked_fits(L[m], q[m])) {
             g_batch_create_error = "model " + std::to_string(m) + ": need 2 <= L <= " +
                                    std::to_string(MAX_L) + " and 2 <= q <= " +
-                                   std::to_string(MAX_Q);
+                                   std::to_string(MAX_Q) +
+                                   ", with chunked couplings under 4 GiB";
             return MI3_EUNSUP;
         }
         if (nwalkers[m] == 0 || nwalkers[m] % 512) {
@@ -480,7 +484,7 @@ int mi3_batch_set_tile(mi3_batch* b, uint32_t model, uint32_t wpb) {
     ClassKey k{state_words(b->L[model]), pf_class(b->q[model])};
     auto opts = tile_options(k.kw);
     if (std::find(opts.begin(), opts.end(), wpb) == opts.end())
-        return bfail(b, MI3_EINVAL, "walkers per tile must be 128, 256 or 512");
+        return bfail(b, MI3_EINVAL, "walkers per tile must be 128, 256 or 512 (128 or 256 for L > 512)");
     hipSetDevice(b->dev);
     BCHK(b, hipStreamSynchronize(b->stream));
     drop_graph(b);

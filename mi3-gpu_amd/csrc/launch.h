@@ -30,7 +30,7 @@ constexpr uint32_t SWEEP_DEFAULT_WPB = 512;
 
 // Largest q / L the register-resident kernels are instantiated for.
 constexpr uint32_t MAX_Q = 25;
-constexpr uint32_t MAX_L = 512;
+constexpr uint32_t MAX_L = 1024;
 
 // Threads per work-group of the sweep / energy kernels.
 constexpr uint32_t POTTS_THREADS = 512;
@@ -48,6 +48,13 @@ inline uint32_t pf_class(uint32_t q) {
 
 // Floats per (padded) chunk.
 inline uint32_t chunk_floats(uint32_t q) { return pf_class(q) * POTTS_THREADS * 4; }
+
+// The chunked couplings of an L-site model are addressed with 32-bit byte
+// offsets (buffer resources): true when they fit.
+inline bool chunked_fits(uint32_t L, uint32_t q) {
+    const uint64_t nchunk = (L + 31) / 32;
+    return (uint64_t)L * nchunk * chunk_floats(q) * 4 < (1ull << 32);
+}
 
 // One model of a batch, as the persistent sweep reads it (device memory).
 struct BatchModel {

@@ -17,7 +17,7 @@ constexpr int NT = 64 * NWAVES;
 // L2 -> LDS traffic per sequence; 32 slots in 8 waves for 2 state words,
 // 16 slots in 16 waves otherwise (measured the faster)
 template <int KW>
-constexpr int eslots() { return KW == 2 ? 32 : 16; }
+constexpr int eslots() { return KW == 2 ? 32 : KW <= 4 ? 16 : 8; }
 template <int KW>
 constexpr int ewaves() { return KW == 2 ? 8 : 16; }
 
@@ -68,6 +68,10 @@ hipError_t dispatch(const A& a, hipStream_t st, uint32_t nchunk, uint32_t q) {
     MI3_CASE(2, 1) MI3_CASE(2, 4) MI3_CASE(2, 7) MI3_CASE(2, 10)
     MI3_CASE(3, 1) MI3_CASE(3, 4) MI3_CASE(3, 7) MI3_CASE(3, 10)
     MI3_CASE(4, 1) MI3_CASE(4, 4) MI3_CASE(4, 7) MI3_CASE(4, 10)
+    MI3_CASE(5, 1) MI3_CASE(5, 4) MI3_CASE(5, 7) MI3_CASE(5, 10)
+    MI3_CASE(6, 1) MI3_CASE(6, 4) MI3_CASE(6, 7) MI3_CASE(6, 10)
+    MI3_CASE(7, 1) MI3_CASE(7, 4) MI3_CASE(7, 7) MI3_CASE(7, 10)
+    MI3_CASE(8, 1) MI3_CASE(8, 4) MI3_CASE(8, 7) MI3_CASE(8, 10)
 #undef MI3_CASE
     return hipErrorInvalidValue;
 }
@@ -76,14 +80,20 @@ template <int KW, int PF>
 struct SweepDisp {
     static hipError_t run(const SweepArgs& a, hipStream_t st) {
         uint32_t w = a.wpb ? a.wpb : SWEEP_DEFAULT_WPB;
-        if (w <= 128) return sweep_kws<KW, PF, 8, 8>(a, st);
-        if (w <= 256) return sweep_kws<KW, PF, 8, 16>(a, st);
-        // four state words do not fit 16 slots in 128 registers; the spills
-        // still pay for q > 8, not for the small chunks of q <= 8
-        if constexpr (KW == 4 && PF == 1)
+        if constexpr (KW >= 5) {
+            // five to eight state words: 8 slots at most fit in 128 registers
+            if (w <= 128) return sweep_kws<KW, PF, 4, 16>(a, st);
             return sweep_kws<KW, PF, 8, 16>(a, st);
-        else
-            return sweep_kws<KW, PF, 16, 16>(a, st);
+        } else {
+            if (w <= 128) return sweep_kws<KW, PF, 8, 8>(a, st);
+            if (w <= 256) return sweep_kws<KW, PF, 8, 16>(a, st);
+            // four state words do not fit 16 slots in 128 registers; the
+            // spills still pay for q > 8, not for the small chunks of q <= 8
+            if constexpr (KW == 4 && PF == 1)
+                return sweep_kws<KW, PF, 8, 16>(a, st);
+            else
+                return sweep_kws<KW, PF, 16, 16>(a, st);
+        }
     }
 };
 template <int KW, int PF>
@@ -113,10 +123,16 @@ hipError_t batch_kws(const BatchModel* models, const BatchTile* tiles, uint32_t 
 template <int KW, int PF>
 hipError_t batch_kw(uint32_t wpb, const BatchModel* models, const BatchTile* tiles,
                     uint32_t ntiles, uint32_t* counter, uint32_t grid, hipStream_t st) {
-    if (wpb == 128) return batch_kws<KW, PF, 8, 8>(models, tiles, ntiles, counter, grid, st);
-    if (wpb == 256) return batch_kws<KW, PF, 8, 16>(models, tiles, ntiles, counter, grid, st);
-    if (wpb == 512) return batch_kws<KW, PF, 16, 16>(models, tiles, ntiles, counter, grid, st);
-    return hipErrorInvalidValue;
+    if constexpr (KW >= 5) {
+        if (wpb == 128) return batch_kws<KW, PF, 4, 16>(models, tiles, ntiles, counter, grid, st);
+        if (wpb == 256) return batch_kws<KW, PF, 8, 16>(models, tiles, ntiles, counter, grid, st);
+        return hipErrorInvalidValue;
+    } else {
+        if (wpb == 128) return batch_kws<KW, PF, 8, 8>(models, tiles, ntiles, counter, grid, st);
+        if (wpb == 256) return batch_kws<KW, PF, 8, 16>(models, tiles, ntiles, counter, grid, st);
+        if (wpb == 512) return batch_kws<KW, PF, 16, 16>(models, tiles, ntiles, counter, grid, st);
+        return hipErrorInvalidValue;
+    }
 }
 }  // namespace
 
@@ -130,6 +146,10 @@ hipError_t launch_sweep_batch(uint32_t kw, uint32_t pf, uint32_t wpb, const Batc
     MI3_BCASE(2, 1) MI3_BCASE(2, 4) MI3_BCASE(2, 7) MI3_BCASE(2, 10)
     MI3_BCASE(3, 1) MI3_BCASE(3, 4) MI3_BCASE(3, 7) MI3_BCASE(3, 10)
     MI3_BCASE(4, 1) MI3_BCASE(4, 4) MI3_BCASE(4, 7) MI3_BCASE(4, 10)
+    MI3_BCASE(5, 1) MI3_BCASE(5, 4) MI3_BCASE(5, 7) MI3_BCASE(5, 10)
+    MI3_BCASE(6, 1) MI3_BCASE(6, 4) MI3_BCASE(6, 7) MI3_BCASE(6, 10)
+    MI3_BCASE(7, 1) MI3_BCASE(7, 4) MI3_BCASE(7, 7) MI3_BCASE(7, 10)
+    MI3_BCASE(8, 1) MI3_BCASE(8, 4) MI3_BCASE(8, 7) MI3_BCASE(8, 10)
 #undef MI3_BCASE
     return hipErrorInvalidValue;
 }

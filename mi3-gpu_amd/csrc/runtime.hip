@@ -244,6 +244,10 @@ int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t s
         return fail(nullptr, MI3_EUNSUP,
                     "this build supports q <= " + std::to_string(MAX_Q) + " and L <= " +
                         std::to_string(MAX_L));
+    if (!chunked_fits(L, q))
+        return fail(nullptr, MI3_EUNSUP,
+                    "L = " + std::to_string(L) + " with q = " + std::to_string(q) +
+                        ": the chunked couplings would exceed 4 GiB");
     if (nwalkers % 512 != 0)
         return fail(nullptr, MI3_EINVAL, "nwalkers/ngpus must be a multiple of 512");
     int ndev = 0;

@@ -26,7 +26,8 @@ def make_gpu(tmp_path, L, q, N, seed=1234, nsteps=None, profile=False, wgsize=No
     return g
 
 
-SHAPES = [(99, 4), (232, 21), (37, 7), (5, 3), (300, 21), (64, 21), (33, 2), (450, 4)]
+SHAPES = [(99, 4), (232, 21), (37, 7), (5, 3), (300, 21), (64, 21), (33, 2), (450, 4),
+          (700, 21), (1000, 4)]
 
 
 @pytest.mark.parametrize("L,q", SHAPES)
@@ -68,7 +69,7 @@ def test_energies_special_cases(gpu_ok, tmp_path):
 
 
 @pytest.mark.parametrize("L,q,N", [(99, 4, 512), (232, 21, 1024), (37, 7, 512), (5, 3, 512),
-                                   (300, 21, 512), (20, 25, 1024)])
+                                   (300, 21, 512), (20, 25, 1024), (700, 21, 512)])
 def test_bicounts_bit_exact(gpu_ok, tmp_path, L, q, N):
     s = seqs_rand(N, L, q, 5)
     g = make_gpu(tmp_path, L, q, N)
@@ -105,7 +106,8 @@ def test_bicounts_ragged_large_buffer(gpu_ok, tmp_path):
                                              (300, 21, 1024, 30, 256), (400, 4, 512, 20, 256),
                                              (500, 4, 512, 12, 128), (350, 21, 1024, 20, 512),
                                              (384, 4, 512, 16, 512), (450, 21, 512, 12, 512),
-                                             (420, 5, 512, 12, 512)])
+                                             (420, 5, 512, 12, 512), (600, 21, 512, 24, 512),
+                                             (700, 21, 512, 24, 256), (1000, 4, 512, 40, 128)])
 def test_sweep_matches_oracle_trajectories(gpu_ok, tmp_path, L, q, N, nsteps, wg):
     J = model(L, q, 7, sj=0.3 if q < 10 else 0.1)
     s0 = seqs_rand(N, L, q, 8)

@@ -51,6 +51,8 @@ def test_create_argument_validation_message():
     assert _lib.lib().mi3_create(0, 10, 4, 500, 1, 0, ctypes.byref(ctx)) == _lib.MI3_EINVAL
     assert b'multiple of 512' in _lib.lib().mi3_last_error(None)
     assert _lib.lib().mi3_create(0, 10, 40, 512, 1, 0, ctypes.byref(ctx)) == _lib.MI3_EUNSUP
+    assert _lib.lib().mi3_create(0, 1025, 21, 512, 1, 0, ctypes.byref(ctx)) == _lib.MI3_EUNSUP
+    assert b'L <= 1024' in _lib.lib().mi3_last_error(None)
     with pytest.raises(ValueError):
         _lib.check(None, _lib.MI3_EINVAL)
 
