@@ -443,7 +443,7 @@ __device__ __forceinline__ void sweep_block(
 
 
 template <int KW, int S, int NWAVES, int PF>
-__global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_sweep(
+__global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(NWAVES / 4, NWAVES / 4))) void k_sweep(
     const float* __restrict__ Ju, uint8_t* __restrict__ seq, const uint32_t* __restrict__ pos_list,
     uint32_t ldseq, uint32_t L, uint32_t q, uint32_t nchunk, uint32_t nsteps, uint32_t k0,
     uint32_t k1, uint64_t step0, uint32_t walker_base, float nbeta) {
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 
 // models: every work-group takes tiles from a shared counter until none is
 // left (the counter is cleared before each launch).
 template <int KW, int S, int NWAVES, int PF>
-__global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(1, 2))) void
+__global__ __launch_bounds__(64 * NWAVES) __attribute__((amdgpu_waves_per_eu(NWAVES / 4, NWAVES / 4))) void
 k_sweep_batch(const BatchModel* __restrict__ models, const BatchTile* __restrict__ tiles,
               uint32_t ntiles, uint32_t* __restrict__ counter) {
     __shared__ uint32_t s_tile;
