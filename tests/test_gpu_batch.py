@@ -8,9 +8,10 @@ from oracle import cboracle
 
 pytestmark = pytest.mark.gpu
 
-# (L, q, walkers): four register classes (L <= 128, <= 256, > 256) x q classes
+# (L, q, walkers): register classes of one to five state words (L <= 128, <= 256,
+# <= 384, > 512) x q classes
 MODELS = [(20, 4, 512), (70, 21, 1024), (140, 4, 512), (33, 21, 512), (260, 4, 512),
-          (99, 21, 512)]
+          (99, 21, 512), (600, 21, 512)]
 
 
 def _setup(seed=40):
@@ -64,7 +65,8 @@ def test_batch_tiles_give_identical_chains(gpu_ok, wpb):
     b1, Js, S0 = _setup()
     b2, _, _ = _setup()
     for m, (L, q, N) in enumerate(MODELS):
-        b2.set_tile(m, wpb)
+        if wpb <= 256 or L <= 512:      # 512-walker tiles hold at most 512 sites
+            b2.set_tile(m, wpb)
     b1.run(2)
     b2.run(2)
     for m in range(len(MODELS)):
@@ -88,6 +90,8 @@ def test_batch_tune_and_errors(gpu_ok):
         b.set_seqs(0, np.full((512, 20), 9, 'u1'))
     with pytest.raises(ValueError):
         b.set_tile(0, 64)
+    with pytest.raises(ValueError):
+        b.set_tile(6, 512)            # L = 600: 128 or 256 walkers per tile
     with pytest.raises(ValueError):
         PottsBatch([10], [4], [100])
     b.close()
