@@ -4,7 +4,9 @@
 // count_{ij}(a, b) = (X^T X)[i*q + a][j*q + b] for i < j.  Each wave owns a
 // 128 x 128 block of the flattened L*q x L*q Gram matrix (4 x 4 tiles of
 // v_mfma_i32_32x32x32_i8), builds its int8 operands on the fly from the
-// site-major residue bytes (16 sequences per lane per k-step), contracts
+// site-major residue bytes (16 sequences per lane per k-step, read from an
+// LDS copy of the block's sites that LDS DMA refills 128 sequences at a
+// time, one load per site and 16 bytes instead of one per row), contracts
 // over its range of sequences with int32 accumulation, and scatters the
 // i < j entries into the packed (L(L-1)/2, q*q) uint32 table.  Integer accumulation makes
 // the result exact whatever the summation order.
@@ -69,11 +71,27 @@ __device__ __forceinline__ uint32_t match_hi(uint32_t v, uint32_t na4) {
     return (v ^ na4) + 0x80808081u;
 }
 
+// Staging of residue bytes: BC_STAGE k-steps (128 sequences, 128 bytes per
+// site) of the sites a block's 128 rows and 128 columns touch (at most
+// 128/q + 2 each), double-buffered.  One LDS DMA instruction moves 16 bytes
+// per lane, i.e. 8 sites of 128 bytes.
+constexpr uint32_t BC_STAGE = 4;
+inline uint32_t bc_max_sites(uint32_t q) { return 128 / q + 2; }
+inline size_t bc_lds_bytes(uint32_t q) {
+    return 2ull * ((2 * bc_max_sites(q) + 7) / 8) * 1024;
+}
+typedef __attribute__((address_space(3))) void bc_lds_void;
+
+// s_waitcnt vmcnt(0): every vector-memory operation of this wave, LDS DMA
+// included, has completed.
+__device__ __forceinline__ void wait_vmem_bc() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restrict__ seq,
                                                          uint32_t ldseq, uint32_t nseq,
                                                          uint32_t L, uint32_t q, uint32_t nblk,
                                                          uint32_t ksplit,
                                                          uint32_t* __restrict__ counts) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t bc_lds[];
     uint32_t t = blockIdx.x / ksplit;
     const uint32_t part = blockIdx.x - t * ksplit;
     uint32_t rb = 0;
@@ -93,19 +111,30 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
         __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(seq), (short)0,
                                           (int)(ldseq * L), 0x00020000);
 
-    uint32_t offA[4], a4A[4], naA[4], offB[4], a4B[4], naB[4];
+    // the sites of this block's rows (A) and columns (B), staged as
+    // [A sites..., B sites...] x 128 bytes per buffer
+    const uint32_t rA0 = rb * 128, rA1 = (rA0 + 127 < Lq) ? rA0 + 127 : Lq - 1;
+    const uint32_t cB0 = cb * 128, cB1 = (cB0 + 127 < Lq) ? cB0 + 127 : Lq - 1;
+    const uint32_t siA0 = rA0 / q, nsA = rA1 / q - siA0 + 1;
+    const uint32_t siB0 = cB0 / q, nsB = cB1 / q - siB0 + 1;
+    const uint32_t ninst = (nsA + nsB + 7) / 8;
+    const uint32_t bufbytes = ninst * 1024;
+
+    uint32_t ldsA[4], a4A[4], naA[4], siteA[4], ldsB[4], a4B[4], naB[4], siteB[4];
 #pragma unroll
     for (int x = 0; x < 4; x++) {
-        const uint32_t r = rb * 128 + x * 32 + lr;
-        const uint32_t si = r < Lq ? r / q : 0;
+        const uint32_t r = rA0 + x * 32 + lr;
+        const uint32_t si = r < Lq ? r / q : siA0;
+        siteA[x] = si;
         a4A[x] = r < Lq ? (r - si * q) * 0x01010101u : 0x7f7f7f7fu;
         naA[x] = ~a4A[x];
-        offA[x] = si * ldseq + 16 * g;
-        const uint32_t c = cb * 128 + x * 32 + lr;
-        const uint32_t sj = c < Lq ? c / q : 0;
+        ldsA[x] = (si - siA0) * 128 + 16 * g;
+        const uint32_t c = cB0 + x * 32 + lr;
+        const uint32_t sj = c < Lq ? c / q : siB0;
+        siteB[x] = sj;
         a4B[x] = c < Lq ? (c - sj * q) * 0x01010101u : 0x7f7f7f7fu;
         naB[x] = ~a4B[x];
-        offB[x] = sj * ldseq + 16 * g;
+        ldsB[x] = (nsA + sj - siB0) * 128 + 16 * g;
     }
     v16i acc[4][4];
 #pragma unroll
@@ -115,15 +144,31 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[x][y][e] = 0;
 
-    auto load = [&](uint32_t step, v4i (&ra)[4], v4i (&rbv)[4]) {
-        const uint32_t n = step * 32;   // wave-uniform: the scalar offset
-#pragma unroll
-        for (int x = 0; x < 4; x++) {
-            ra[x] = __builtin_amdgcn_raw_buffer_load_b128(rs, offA[x], n, 0);
-            rbv[x] = __builtin_amdgcn_raw_buffer_load_b128(rs, offB[x], n, 0);
+    // LDS DMA of one stage: instruction k moves sites 8k .. 8k+7 (lane l:
+    // site 8k + l/8, 16-byte piece l%8 of the stage's 128 sequences);
+    // slots past the block's sites load a valid site that nothing reads.
+    // The last stage of a range can reach past the last step (and, for the
+    // last site, past the buffer): such pieces are never read, and their
+    // source offsets are clamped into the buffer.
+    const uint32_t last16 = ldseq * L - 16;
+    auto stage = [&](uint32_t buf, uint32_t step) {
+        const uint32_t n0 = step * 32;
+        for (uint32_t k = 0; k < ninst; k++) {
+            const uint32_t ts = 8 * k + (l >> 3);
+            const uint32_t site = ts < nsA ? siA0 + ts : (ts < nsA + nsB ? siB0 + (ts - nsA) : siA0);
+            const uint32_t off = site * ldseq + n0 + 16 * (l & 7);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rs, (bc_lds_void*)(bc_lds + buf * bufbytes + k * 1024), 16,
+                off < last16 ? off : last16, 0, 0, 0);
         }
     };
-    auto operands = [&](const v4i (&ra)[4], const v4i (&rbv)[4], v4i (&A)[4], v4i (&B)[4]) {
+    auto operands = [&](const uint8_t* sbase, uint32_t ks, v4i (&A)[4], v4i (&B)[4]) {
+        v4i ra[4], rbv[4];
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            ra[x] = *reinterpret_cast<const v4i*>(sbase + ldsA[x] + 32 * ks);
+            rbv[x] = *reinterpret_cast<const v4i*>(sbase + ldsB[x] + 32 * ks);
+        }
 #pragma unroll
         for (int x = 0; x < 4; x++)
 #pragma unroll
@@ -139,31 +184,41 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
             for (int y = 0; y < 4; y++)
                 acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[x], B[y], acc[x][y], 0, 0, 0);
     };
-    // full steps (all 32 sequences < nseq), two buffers of raw bytes loaded
-    // two steps ahead; the ragged last step, if any, goes byte by byte
+    // full steps (all 32 sequences < nseq), staged BC_STAGE steps at a time
+    // one stage ahead; the ragged last step, if any, goes byte by byte
     const uint32_t full_end = nseq / 32;
     const uint32_t e1 = s1 < full_end ? s1 : full_end;
-    v4i r0a[4], r0b[4], r1a[4], r1b[4], A[4], B[4];
-    if (s0 < e1) load(s0, r0a, r0b);
-    if (s0 + 1 < e1) load(s0 + 1, r1a, r1b);
-    for (uint32_t st = s0; st < e1; st += 2) {
-        operands(r0a, r0b, A, B);
-        if (st + 2 < e1) load(st + 2, r0a, r0b);
-        products(A, B);
-        if (st + 1 < e1) {
-            operands(r1a, r1b, A, B);
-            if (st + 3 < e1) load(st + 3, r1a, r1b);
-            products(A, B);
+    if (s0 < e1) {
+        stage(0, s0);
+        uint32_t buf = 0;
+        for (uint32_t st0 = s0; st0 < e1; st0 += BC_STAGE) {
+            wait_vmem_bc();
+            __syncthreads();   // this stage's bytes are in LDS
+            if (st0 + BC_STAGE < e1) stage(buf ^ 1, st0 + BC_STAGE);
+            const uint8_t* sbase = bc_lds + buf * bufbytes;
+            // one step at a time (a guard per step also keeps the compiler
+            // from hoisting all the stage's LDS reads, which would spill)
+#pragma unroll
+            for (uint32_t ks = 0; ks < BC_STAGE; ks++) {
+                if (st0 + ks < e1) {
+                    v4i A[4], B[4];
+                    operands(sbase, ks, A, B);
+                    products(A, B);
+                }
+            }
+            __syncthreads();   // reads of this buffer done before it is refilled
+            buf ^= 1;
         }
     }
     if (s1 > full_end && full_end >= s0) {
         // ragged tail: the last, partial step of this part (bytes past nseq
         // count for nothing)
         const uint32_t n = full_end * 32 + 16 * g;
+        v4i A[4], B[4];
 #pragma unroll
         for (int x = 0; x < 4; x++) {
-            v4i ra = onehot16(seq, ldseq, offA[x] / ldseq, a4A[x], a4A[x] != 0x7f7f7f7fu, n, nseq);
-            v4i rbv = onehot16(seq, ldseq, offB[x] / ldseq, a4B[x], a4B[x] != 0x7f7f7f7fu, n, nseq);
+            v4i ra = onehot16(seq, ldseq, siteA[x], a4A[x], a4A[x] != 0x7f7f7f7fu, n, nseq);
+            v4i rbv = onehot16(seq, ldseq, siteB[x], a4B[x], a4B[x] != 0x7f7f7f7fu, n, nseq);
             // onehot16 gives 0/1 bytes; rows take -128
 #pragma unroll
             for (int e = 0; e < 4; e++) ra[e] = (int)((uint32_t)ra[e] << 7);

@@ -687,6 +687,8 @@ int mi3_calc_bicounts(mi3_ctx* c, const char* seqbuf, uint32_t nseq) {
     size_t ld = seq_ld(c, sb);
     if (nseq > ld) return fail(c, MI3_EINVAL, "nseq exceeds the sequence buffer");
     if (ld % 16) return fail(c, MI3_EINVAL, "sequence buffer row stride must be a multiple of 16");
+    if ((uint64_t)ld * c->L >= (1ull << 31))
+        return fail(c, MI3_EUNSUP, "bicounts: sequence buffer of 2 GiB or more");
     rc = need(c, "bicount", (size_t)c->nP * c->q * c->q * 4, &cb);
     if (rc) return rc;
     const uint32_t Lq = c->L * c->q;
@@ -707,9 +709,9 @@ int mi3_calc_bicounts(mi3_ctx* c, const char* seqbuf, uint32_t nseq) {
         int frc = fill_u32_dev(c, cb->d, 0u, (uint64_t)c->nP * c->q * c->q);
         if (frc) return frc;
     }
-    hipLaunchKernelGGL(k_bicounts_mfma128, dim3(nbw * ksplit), dim3(64), 0, c->stream,
-                       (const uint8_t*)sb->d, (uint32_t)ld, nseq, c->L, c->q, nblk, ksplit,
-                       (uint32_t*)cb->d);
+    hipLaunchKernelGGL(k_bicounts_mfma128, dim3(nbw * ksplit), dim3(64), bc_lds_bytes(c->q),
+                       c->stream, (const uint8_t*)sb->d, (uint32_t)ld, nseq, c->L, c->q, nblk,
+                       ksplit, (uint32_t*)cb->d);
     HIPCHK(c, hipGetLastError());
     cb->version++;
     return MI3_OK;
