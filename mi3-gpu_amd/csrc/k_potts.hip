@@ -54,6 +54,11 @@ __device__ __forceinline__ uint32_t state_word(const uint32_t (&w)[KW], uint32_t
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
 
+// Sweep gather schedule: slots per group and groups of reads in flight
+// (8 and 1, 8 and 2, 4 and 2, 4 and 3, 16 and 1 time alike at L=232 q=21).
+constexpr int SWEEP_G = 8;
+constexpr int SWEEP_P = 1;
+
 // Wave-wide lane masks: bit `lane` set when (lane & m) != 0.
 __device__ __forceinline__ constexpr uint64_t lane_bit_mask(int m) {
     return m == 16 ? 0xFFFF0000FFFF0000ull
@@ -357,13 +362,13 @@ __device__ __forceinline__ void sweep_block(
             for (int g = 0; g < GB; g++) {
                 baseA[s0 + g] = mad_u24(a[g], qch4, j4);
                 baseB[s0 + g] = b[g] + j4;
-                acc[s0 + g] = 0.f;
             }
         }
         // stream the couplings of site pos, chunk by chunk
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            if ((uint32_t)k < nchunk) {
+            // (chunk 0 always exists: nchunk >= 1)
+            if (k == 0 || (uint32_t)k < nchunk) {
                 const bool last_chunk = (uint32_t)k + 1 == nchunk;
                 const bool has_next = !last_chunk || (t + 1 < nsteps);
                 if (has_next) {
@@ -374,23 +379,38 @@ __device__ __forceinline__ void sweep_block(
                                              ldsb + (buf ^ 1) * chb, wave);
                 }
                 const char* cb = ldsb + buf * chb;
-                // gathers issued in groups of G slots ahead of their use, so
-                // that one LDS wait covers 2*G reads
-                constexpr int G = (S < 8) ? S : 8;
+                // gathers in groups of G slots, P groups in flight: the reads
+                // of group i are issued before the sums of group i - P + 1,
+                // and scheduling barriers keep the compiler from interleaving
+                // them slot by slot (which exposes the LDS latency per slot)
+                constexpr int G = (S < SWEEP_G) ? S : SWEEP_G;
+                constexpr int NG = S / G;
+                constexpr int P = (NG < SWEEP_P) ? NG : SWEEP_P;
+                float vA[S], vB[S];
 #pragma unroll
-                for (int s0 = 0; s0 < S; s0 += G) {
-                    float vA[G], vB[G];
+                for (int i = 0; i < NG + P - 1; i++) {
+                    if (i < NG) {
 #pragma unroll
-                    for (int g = 0; g < G; g++) {
-                        const int s = s0 + g;
-                        static_assert(CH * 4 == 128, "row stride of a chunk");
-                        const uint32_t off =
-                            byte_x128(state_word<KW>(st[s], (uint32_t)k >> 2), k & 3);
-                        vA[g] = *reinterpret_cast<const float*>(cb + baseA[s] + off);
-                        vB[g] = *reinterpret_cast<const float*>(cb + baseB[s] + off);
+                        for (int g = 0; g < G; g++) {
+                            const int s = i * G + g;
+                            static_assert(CH * 4 == 128, "row stride of a chunk");
+                            const uint32_t off =
+                                byte_x128(state_word<KW>(st[s], (uint32_t)k >> 2), k & 3);
+                            vA[s] = *reinterpret_cast<const float*>(cb + baseA[s] + off);
+                            vB[s] = *reinterpret_cast<const float*>(cb + baseB[s] + off);
+                        }
                     }
+                    __builtin_amdgcn_sched_barrier(0);
+                    if (i >= P - 1) {
 #pragma unroll
-                    for (int g = 0; g < G; g++) acc[s0 + g] += vB[g] - vA[g];
+                        for (int g = 0; g < G; g++) {
+                            const int s = (i - P + 1) * G + g;
+                            // the first chunk starts the sums
+                            if (k == 0) acc[s] = vB[s] - vA[s];
+                            else acc[s] += vB[s] - vA[s];
+                        }
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
                 wait_vmem();
                 __syncthreads();
