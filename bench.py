@@ -52,7 +52,7 @@ HBM_PEAK_GBS = 8000.0
 # device-to-device copy in the same run (FETCH_SIZE x2.0, WRITE_SIZE x1.0);
 # summary in profiles/k_sweep_L232_q21_N262144_pmc_traffic.md.  Keyed by
 # (L, q, walkers, sweeps per launch, walkers per work-group).
-MEASURED_TRAFFIC = {(232, 21, 2 ** 18, 8, 512): 13.39e9}
+MEASURED_TRAFFIC = {(232, 21, 2 ** 18, 8, 512): 13.41e9}
 
 
 def production_model(L=232, q=21, seed=1, hs=1.0, js=0.05):
