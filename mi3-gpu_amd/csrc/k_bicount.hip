@@ -5,7 +5,7 @@
 // 128 x 128 block of the flattened L*q x L*q Gram matrix (4 x 4 tiles of
 // v_mfma_i32_32x32x32_i8), builds its int8 operands on the fly from the
 // site-major residue bytes (16 sequences per lane per k-step, read from an
-// LDS copy of the block's sites that LDS DMA refills 128 sequences at a
+// LDS copy of the block's sites that LDS DMA refills 4 to 16 k-steps at a
 // time, one load per site and 16 bytes instead of one per row), contracts
 // over its range of sequences with int32 accumulation, and scatters the
 // i < j entries into the packed (L(L-1)/2, q*q) uint32 table.  Integer accumulation makes
@@ -71,14 +71,29 @@ __device__ __forceinline__ uint32_t match_hi(uint32_t v, uint32_t na4) {
     return (v ^ na4) + 0x80808081u;
 }
 
-// Staging of residue bytes: BC_STAGE k-steps (128 sequences, 128 bytes per
-// site) of the sites a block's 128 rows and 128 columns touch (at most
-// 128/q + 2 each), double-buffered.  One LDS DMA instruction moves 16 bytes
-// per lane, i.e. 8 sites of 128 bytes.
-constexpr uint32_t BC_STAGE = 4;
+// Staging of residue bytes: STAGE k-steps (32 sequences each, so
+// SB = 32 * STAGE bytes per site) of the sites a block's 128 rows and 128
+// columns touch (at most 128/q + 2 each), double-buffered.  One LDS DMA
+// instruction moves 16 bytes per lane, i.e. SPI = 1024 / SB sites.  Longer
+// stages wait less often on their copies (16 k-steps: 10.9 ms, 8: 11.1,
+// 4: 11.5, 2: 12.4 at 2^20 sequences, L=232 q=21); the host takes the
+// longest whose double buffer fits 40 KiB, so that four waves share a CU.
+template <uint32_t STAGE>
+struct BcStage {
+    static constexpr uint32_t SB = 32 * STAGE;
+    static constexpr uint32_t LPS = SB / 16;   // lanes per site
+    static constexpr uint32_t SPI = 64 / LPS;  // sites per instruction
+    static_assert(SPI >= 1 && SPI * LPS == 64, "stage size");
+};
 inline uint32_t bc_max_sites(uint32_t q) { return 128 / q + 2; }
-inline size_t bc_lds_bytes(uint32_t q) {
-    return 2ull * ((2 * bc_max_sites(q) + 7) / 8) * 1024;
+inline size_t bc_lds_bytes(uint32_t q, uint32_t stage) {
+    const uint32_t spi = 32 / stage;
+    return 2ull * ((2 * bc_max_sites(q) + spi - 1) / spi) * 1024;
+}
+inline uint32_t bc_stage_for(uint32_t q) {
+    for (uint32_t st : {16u, 8u})
+        if (bc_lds_bytes(q, st) <= 40960) return st;
+    return 4;
 }
 typedef __attribute__((address_space(3))) void bc_lds_void;
 
@@ -86,12 +101,15 @@ typedef __attribute__((address_space(3))) void bc_lds_void;
 // included, has completed.
 __device__ __forceinline__ void wait_vmem_bc() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+template <uint32_t STAGE>
 __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restrict__ seq,
                                                          uint32_t ldseq, uint32_t nseq,
                                                          uint32_t L, uint32_t q, uint32_t nblk,
                                                          uint32_t ksplit,
                                                          uint32_t* __restrict__ counts) {
     extern __shared__ __attribute__((aligned(16))) uint8_t bc_lds[];
+    constexpr uint32_t BC_STAGE = STAGE, BC_SB = BcStage<STAGE>::SB,
+                       BC_LPS = BcStage<STAGE>::LPS, BC_SPI = BcStage<STAGE>::SPI;
     uint32_t t = blockIdx.x / ksplit;
     const uint32_t part = blockIdx.x - t * ksplit;
     uint32_t rb = 0;
@@ -112,12 +130,12 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
                                           (int)(ldseq * L), 0x00020000);
 
     // the sites of this block's rows (A) and columns (B), staged as
-    // [A sites..., B sites...] x 128 bytes per buffer
+    // [A sites..., B sites...] x BC_SB bytes per buffer
     const uint32_t rA0 = rb * 128, rA1 = (rA0 + 127 < Lq) ? rA0 + 127 : Lq - 1;
     const uint32_t cB0 = cb * 128, cB1 = (cB0 + 127 < Lq) ? cB0 + 127 : Lq - 1;
     const uint32_t siA0 = rA0 / q, nsA = rA1 / q - siA0 + 1;
     const uint32_t siB0 = cB0 / q, nsB = cB1 / q - siB0 + 1;
-    const uint32_t ninst = (nsA + nsB + 7) / 8;
+    const uint32_t ninst = (nsA + nsB + BC_SPI - 1) / BC_SPI;
     const uint32_t bufbytes = ninst * 1024;
 
     uint32_t ldsA[4], a4A[4], naA[4], siteA[4], ldsB[4], a4B[4], naB[4], siteB[4];
@@ -128,13 +146,13 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
         siteA[x] = si;
         a4A[x] = r < Lq ? (r - si * q) * 0x01010101u : 0x7f7f7f7fu;
         naA[x] = ~a4A[x];
-        ldsA[x] = (si - siA0) * 128 + 16 * g;
+        ldsA[x] = (si - siA0) * BC_SB + 16 * g;
         const uint32_t c = cB0 + x * 32 + lr;
         const uint32_t sj = c < Lq ? c / q : siB0;
         siteB[x] = sj;
         a4B[x] = c < Lq ? (c - sj * q) * 0x01010101u : 0x7f7f7f7fu;
         naB[x] = ~a4B[x];
-        ldsB[x] = (nsA + sj - siB0) * 128 + 16 * g;
+        ldsB[x] = (nsA + sj - siB0) * BC_SB + 16 * g;
     }
     v16i acc[4][4];
 #pragma unroll
@@ -144,8 +162,9 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[x][y][e] = 0;
 
-    // LDS DMA of one stage: instruction k moves sites 8k .. 8k+7 (lane l:
-    // site 8k + l/8, 16-byte piece l%8 of the stage's 128 sequences);
+    // LDS DMA of one stage: instruction k moves sites BC_SPI*k onward (lane
+    // l: site BC_SPI*k + l/BC_LPS, 16-byte piece l%BC_LPS of the stage's
+    // 32*BC_STAGE sequences);
     // slots past the block's sites load a valid site that nothing reads.
     // The last stage of a range can reach past the last step (and, for the
     // last site, past the buffer): such pieces are never read, and their
@@ -154,9 +173,9 @@ __global__ __launch_bounds__(64) void k_bicounts_mfma128(const uint8_t* __restri
     auto stage = [&](uint32_t buf, uint32_t step) {
         const uint32_t n0 = step * 32;
         for (uint32_t k = 0; k < ninst; k++) {
-            const uint32_t ts = 8 * k + (l >> 3);
+            const uint32_t ts = BC_SPI * k + l / BC_LPS;
             const uint32_t site = ts < nsA ? siA0 + ts : (ts < nsA + nsB ? siB0 + (ts - nsA) : siA0);
-            const uint32_t off = site * ldseq + n0 + 16 * (l & 7);
+            const uint32_t off = site * ldseq + n0 + 16 * (l % BC_LPS);
             __builtin_amdgcn_raw_ptr_buffer_load_lds(
                 rs, (bc_lds_void*)(bc_lds + buf * bufbytes + k * 1024), 16,
                 off < last16 ? off : last16, 0, 0, 0);

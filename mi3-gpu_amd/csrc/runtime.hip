@@ -709,9 +709,12 @@ int mi3_calc_bicounts(mi3_ctx* c, const char* seqbuf, uint32_t nseq) {
         int frc = fill_u32_dev(c, cb->d, 0u, (uint64_t)c->nP * c->q * c->q);
         if (frc) return frc;
     }
-    hipLaunchKernelGGL(k_bicounts_mfma128, dim3(nbw * ksplit), dim3(64), bc_lds_bytes(c->q),
-                       c->stream, (const uint8_t*)sb->d, (uint32_t)ld, nseq, c->L, c->q, nblk,
-                       ksplit, (uint32_t*)cb->d);
+    const uint32_t stage = bc_stage_for(c->q);
+    auto kern = stage == 16 ? k_bicounts_mfma128<16> : stage == 8 ? k_bicounts_mfma128<8>
+                                                                   : k_bicounts_mfma128<4>;
+    hipLaunchKernelGGL(kern, dim3(nbw * ksplit), dim3(64), bc_lds_bytes(c->q, stage), c->stream,
+                       (const uint8_t*)sb->d, (uint32_t)ld, nseq, c->L, c->q, nblk, ksplit,
+                       (uint32_t*)cb->d);
     HIPCHK(c, hipGetLastError());
     cb->version++;
     return MI3_OK;
