@@ -184,12 +184,20 @@ int mi3_calc_energies(mi3_ctx *ctx, const char *seqbuf, const char *Jbuf, const 
  * (mcmc.cl:464-516). */
 int mi3_calc_bicounts(mi3_ctx *ctx, const char *seqbuf, uint32_t nseq);
 
+/* Number of parts mi3_calc_bicounts splits nseq sequences into (partial
+ * counts of the parts are added with integer atomics when it is above 1).
+ * A query of the launch plan behind calcBicounts (mcmcGPU.py:501-518); the
+ * reference's countBivariate has no split (mcmc.cl:464-516). */
+int mi3_bicount_split(mi3_ctx *ctx, uint32_t nseq, uint32_t *ksplit);
+
 /* "bi" = float("bicount") / float(nseq) (bicounts_to_bimarg,
  * mcmcGPU.py:520-534, mcmc.cl:518-528). */
 int mi3_bicounts_to_bimarg(mi3_ctx *ctx, uint32_t nseq);
 
 /* "bi"[pair][a*q+b] = sum_n wbuf[n] [s_i=a][s_j=b] over the first nseq
- * sequences (weightedMarg, mcmcGPU.py:634-661, mcmc.cl:646-718). */
+ * sequences (weightedMarg, mcmcGPU.py:634-661, mcmc.cl:646-718).  Summed
+ * exactly in 64-bit fixed point of sum(w) * 2^-62, then rounded to float
+ * once, so the result does not depend on summation order. */
 int mi3_weighted_marg(mi3_ctx *ctx, const char *seqbuf, const char *wbuf, uint32_t nseq);
 
 /* Divide each pair block of "bi" by its sum (renormalize_bimarg,

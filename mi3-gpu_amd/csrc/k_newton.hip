@@ -202,116 +202,152 @@ __global__ void k_add_floats(float* __restrict__ dst, const float* __restrict__ 
 // Weighted pair histograms F_{im}(a,b) = sum_n w_n [s_ni = a][s_nm = b]
 // (weightedMarg, mcmc.cl:646-718), unnormalised.
 //
-// Accumulation is exact integer arithmetic: every weight is first converted
-// to 64-bit fixed point, wq_n = round(w_n * 2^62 / sum(w)) (so no bin can
-// exceed 2^62), bins are summed with LDS ds_add_u64 (the LDS unit executes
-// 64-bit integer adds at full rate, float adds ~20x slower), and the result
-// is scaled back to float once.  The rounding error per term is below
-// sum(w) * 2^-63, far under float32 summation error, and the result does
-// not depend on the order of the additions.
+// Accumulation is exact integer arithmetic in 64-bit fixed point: every
+// weight becomes wq_n = round(w_n * 2^62 / sum(w)), so that no bin of any
+// subset of sequences can exceed 2^63; bins are summed with LDS ds_add_u64
+// (integer LDS adds run about 20x faster than float ones), parts of the
+// sequence range are added with 64-bit integer atomics, and the result is
+// scaled back to float once.  The rounding error per term is below
+// 2^-63 sum(w) (plus the 2^-53 relative rounding of the double product),
+// so every bin, even one holding a single sequence of a 2^24-sequence set,
+// is the correctly rounded float of its exact weighted sum to within an ulp,
+// and the result does not depend on the order of any addition.
 //
-// Pass 1: fixed-point weights wq = round(w * 2^31 / sum(w)), so that no bin
-// of any subset of sequences can exceed 2^31 (a non-finite or zero total
-// makes every output NaN, which keeps the divergence check of the Newton
-// loop working).
+// Pass 1: fixed-point weights (a non-finite or zero total makes every output
+// NaN, which keeps the divergence check of the Newton loop working).  Within
+// each whole group of 16 sequences the even-numbered weights are stored
+// first and the odd-numbered ones after them, so that each half-wave of the
+// histogram kernel reads its 8 weights as contiguous vector loads; the tail
+// past the last whole group keeps its natural order.
 __global__ void k_weights_fixed(const float* __restrict__ w, uint32_t n,
-                                const float* __restrict__ wsum, uint32_t* __restrict__ wq,
+                                const float* __restrict__ wsum, uint64_t* __restrict__ wq,
                                 uint32_t* __restrict__ bad) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const double tot = (double)wsum[0];
     const bool ok = tot > 0.0 && tot < INFINITY;
     if (i == 0) bad[0] = ok ? 0u : 1u;
     if (i >= n) return;
-    double v = ok ? (double)w[i] * (2147483648.0 / tot) : 0.0;  // 2^31 / sum
-    wq[i] = (v > 0.0) ? (uint32_t)(v + 0.5) : 0u;
+    const double v = ok ? (double)w[i] * (4611686018427387904.0 / tot) : 0.0;  // 2^62 / sum
+    const uint32_t r = i & 15u;
+    const uint32_t dst = (i < (n & ~15u)) ? (i & ~15u) + ((r & 1u) ? 8u : 0u) + (r >> 1) : i;
+    wq[dst] = (v > 0.0) ? (uint64_t)(v + 0.5) : 0ull;
 }
 
-// Pass 2: one work-group per (row i, group of 64 partner sites m > i, part
-// of the sequence range).  The 64 pair histograms live in LDS as uint32
-// [a*q+b][64]: lane l of every wave owns pair (i, 64*cp + l), and all 64
-// lanes of a wave add the same sequence, so the row-i residues and the
-// weights are wave-uniform loads and one wave's adds move 256 bytes.  The
-// 16 waves take groups of 16 sequences in turn, loading the next group
-// before adding the current one.  The part's histograms are added into the
-// uint32 accumulator (zeroed beforehand) with integer atomics, so the
-// result does not depend on the order of any addition.
+// Pass 2: one work-group per (row i, group of 32 partner sites m > i, part
+// of the sequence range).  The 32 pair histograms live in LDS as uint64
+// [a*q+b][32]: lane l of a wave owns pair (i, 32*cp + (l & 31)), and the two
+// half-waves add two consecutive sequences (the lower half the even one),
+// so the row-i residues are wave-uniform (scalar) loads.  The 16 waves take
+// groups of 16 sequences in turn, two groups per iteration with all their
+// loads issued before the first add.  The part's histograms are added into
+// the uint64 accumulator (zeroed beforehand) with integer atomics.
 constexpr uint32_t WM_THREADS = 1024;
-constexpr uint32_t WM_COLS = 64;
+constexpr uint32_t WM_COLS = 32;
+
+struct WmGroup {
+    uint4 sm;        // partner-site residues of the 16 sequences
+    uint4 si;        // row-i residues (wave-uniform)
+    uint64_t w[8];   // this half-wave's weights
+};
+
+__device__ __forceinline__ void wm_load(WmGroup& G, const uint8_t* rowi, const uint8_t* rowm,
+                                        const uint64_t* wq, uint32_t g, bool upper) {
+    G.sm = *reinterpret_cast<const uint4*>(rowm + g * 16);
+    G.si = *reinterpret_cast<const uint4*>(rowi + g * 16);
+    const ulonglong2* w2 = reinterpret_cast<const ulonglong2*>(wq + (size_t)g * 16 + (upper ? 8 : 0));
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const ulonglong2 v = w2[k];
+        G.w[2 * k] = v.x;
+        G.w[2 * k + 1] = v.y;
+    }
+}
+
+__device__ __forceinline__ void wm_add(const WmGroup& G, unsigned long long* hist,
+                                       uint32_t qcol8, uint32_t col8, uint32_t sh) {
+    // the upper half-wave takes the odd sequence of each pair: byte offsets
+    // of its residues are one higher
+    const uint32_t si[4] = {G.si.x, G.si.y, G.si.z, G.si.w};
+    const uint32_t sm[4] = {G.sm.x, G.sm.y, G.sm.z, G.sm.w};
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const uint32_t off = sh + 16u * (e & 1);
+        const uint32_t a = __builtin_amdgcn_ubfe(si[e >> 1], off, 8);
+        const uint32_t b = __builtin_amdgcn_ubfe(sm[e >> 1], off, 8);
+        const uint32_t addr = __umul24(a, qcol8) + b * (WM_COLS * 8) + col8;
+        atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(hist) + addr),
+                  (unsigned long long)G.w[e]);
+    }
+}
 
 __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __restrict__ seq,
                                                               uint32_t ldseq, uint32_t nseq,
-                                                              const uint32_t* __restrict__ wq,
+                                                              const uint64_t* __restrict__ wq,
                                                               uint32_t L, uint32_t q,
                                                               const uint32_t* __restrict__ units,
                                                               uint32_t nsplit,
-                                                              uint32_t* __restrict__ acc) {
-    extern __shared__ uint32_t hist32[];
+                                                              unsigned long long* __restrict__ acc) {
+    extern __shared__ unsigned long long hist64[];
     constexpr uint32_t NW = WM_THREADS / 64;
     const uint32_t unit = units[blockIdx.x / nsplit], part = blockIdx.x % nsplit;
     const uint32_t i = unit >> 8, cp = unit & 0xff;
     const uint32_t qq = q * q;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t e = tid; e < qq * WM_COLS; e += WM_THREADS) hist32[e] = 0u;
+    for (uint32_t e = tid; e < qq * WM_COLS; e += WM_THREADS) hist64[e] = 0ull;
     __syncthreads();
-    const uint32_t lane = tid & 63;
+    const uint32_t lane = tid & 63, col = lane & (WM_COLS - 1);
+    const bool upper = lane >= 32;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
-    const uint32_t m = cp * WM_COLS + lane;
+    const uint32_t m = cp * WM_COLS + col;
     const bool valid = (m > i) && (m < L);
     const uint8_t* rowi = seq + (size_t)i * ldseq;
     const uint8_t* rowm = seq + (size_t)(valid ? m : i) * ldseq;
-    const uint32_t qcol = q * WM_COLS;
+    const uint32_t qcol8 = q * WM_COLS * 8, col8 = col * 8, sh = upper ? 8u : 0u;
     // this part's groups of 16 sequences
     const uint32_t ngroups = nseq / 16;
     const uint32_t per = (ngroups + nsplit - 1) / nsplit;
     const uint32_t g0 = part * per;
     const uint32_t g1 = (g0 + per < ngroups) ? g0 + per : ngroups;
     uint32_t g = g0 + wave;
-    uint4 sm_cur = make_uint4(0, 0, 0, 0);
-    if (g < g1) sm_cur = *reinterpret_cast<const uint4*>(rowm + g * 16);
-    while (g < g1) {
-        const uint32_t gn = g + NW;
-        uint4 sm_nxt = make_uint4(0, 0, 0, 0);
-        if (gn < g1) sm_nxt = *reinterpret_cast<const uint4*>(rowm + gn * 16);
-        const uint4 si4 = *reinterpret_cast<const uint4*>(rowi + g * 16);
-        const uint4* w4 = reinterpret_cast<const uint4*>(wq + g * 16);
-        const uint4 wv[4] = {w4[0], w4[1], w4[2], w4[3]};
+    for (; g + NW < g1; g += 2 * NW) {
+        WmGroup A, B;
+        wm_load(A, rowi, rowm, wq, g, upper);
+        wm_load(B, rowi, rowm, wq, g + NW, upper);
         if (valid) {
-            const uint32_t si[4] = {si4.x, si4.y, si4.z, si4.w};
-            const uint32_t sm[4] = {sm_cur.x, sm_cur.y, sm_cur.z, sm_cur.w};
-#pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const uint32_t a = (si[e >> 2] >> (8 * (e & 3))) & 0xffu;
-                const uint32_t b = (sm[e >> 2] >> (8 * (e & 3))) & 0xffu;
-                const uint4 w = wv[e >> 2];
-                const uint32_t v = (e & 3) == 0 ? w.x : (e & 3) == 1 ? w.y : (e & 3) == 2 ? w.z : w.w;
-                atomicAdd(&hist32[__umul24(a, qcol) + b * WM_COLS + lane], v);
-            }
+            wm_add(A, hist64, qcol8, col8, sh);
+            wm_add(B, hist64, qcol8, col8, sh);
         }
-        sm_cur = sm_nxt;
-        g = gn;
     }
-    // the last nseq % 16 sequences, by the first wave of the last part
-    if (part + 1 == nsplit && wave == 0 && valid) {
+    if (g < g1) {
+        WmGroup A;
+        wm_load(A, rowi, rowm, wq, g, upper);
+        if (valid) wm_add(A, hist64, qcol8, col8, sh);
+    }
+    // the last nseq % 16 sequences (weights in natural order), by the lower
+    // half of the first wave of the last part
+    if (part + 1 == nsplit && wave == 0 && valid && !upper) {
         for (uint32_t n = ngroups * 16; n < nseq; n++)
-            atomicAdd(&hist32[__umul24(rowi[n], qcol) + rowm[n] * WM_COLS + lane], wq[n]);
+            atomicAdd(&hist64[__umul24(rowi[n], q * WM_COLS) + rowm[n] * WM_COLS + col],
+                      (unsigned long long)wq[n]);
     }
     __syncthreads();
     for (uint32_t e = tid; e < qq * WM_COLS; e += WM_THREADS) {
-        const uint32_t col = e & (WM_COLS - 1), ab = e / WM_COLS;
-        const uint32_t mm = cp * WM_COLS + col;
-        const uint32_t v = hist32[e];
+        const uint32_t c = e & (WM_COLS - 1), ab = e / WM_COLS;
+        const uint32_t mm = cp * WM_COLS + c;
+        const unsigned long long v = hist64[e];
         if (mm > i && mm < L && v) atomicAdd(&acc[(size_t)pair_index(i, mm, L) * qq + ab], v);
     }
 }
 
-// Pass 3, in place: bins of 2^-31 sum(w) to float (NaN if the total was bad).
-__global__ void k_fixed_to_float(uint32_t* __restrict__ acc, uint64_t n,
+// Pass 3: bins of 2^-62 sum(w) to float (NaN if the total was bad).
+__global__ void k_fixed_to_float(const unsigned long long* __restrict__ acc,
+                                 float* __restrict__ out, uint64_t n,
                                  const float* __restrict__ wsum,
                                  const uint32_t* __restrict__ bad) {
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    const double back = bad[0] ? NAN : (double)wsum[0] * (1.0 / 2147483648.0);
-    reinterpret_cast<float*>(acc)[idx] = (float)((double)acc[idx] * back);
+    const double back = bad[0] ? NAN : (double)wsum[0] * (1.0 / 4611686018427387904.0);
+    out[idx] = (float)((double)acc[idx] * back);
 }
 
 // The other regularisers of the coupling step (mcmc.cl:904-1243), one wave

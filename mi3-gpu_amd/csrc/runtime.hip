@@ -86,8 +86,9 @@ struct mi3_ctx {
     std::map<std::string, Chunked> chunked;  // chunked layouts keyed by source buffer
     double* red = nullptr;                    // reduction partials
     float* red_out = nullptr;                 // 2 floats: internal reduction result
-    uint32_t* wq = nullptr;                   // fixed-point weights
+    uint64_t* wq = nullptr;                   // fixed-point weights
     size_t wq_cap = 0;
+    unsigned long long* wacc = nullptr;       // fixed-point weighted-marginal bins
     uint32_t* wbad = nullptr;
     uint32_t* wm_units = nullptr;
     uint32_t n_units = 0;
@@ -207,6 +208,26 @@ int pin_get(mi3_ctx* c, size_t bytes) {
     return (int)c->pins.size() - 1;
 }
 
+// Parts of the sequence range for the bicount kernel: about four waves per
+// SIMD (one wave fits per SIMD at a time) share the work evenly, keeping at
+// least 64 k-steps of 32 sequences per wave; each wave's int32 accumulators
+// reach -128 per co-occurrence, so a part stays below 2^24 sequences
+// (|acc| < 2^31).
+uint32_t bicount_split(const mi3_ctx* c, uint32_t nseq) {
+    const uint32_t Lq = c->L * c->q;
+    const uint32_t nblk = (Lq + 127) / 128;
+    const uint32_t nbw = nblk * (nblk + 1) / 2;
+    const uint32_t ksteps = (nseq + 31) / 32;
+    const uint32_t target = 4 * 4 * c->ncu;
+    uint32_t ksplit = (target + nbw / 2) / nbw;
+    const uint32_t kmax = ksteps / 64 > 1 ? ksteps / 64 : 1;
+    if (ksplit > kmax) ksplit = kmax;
+    if (ksplit > 32) ksplit = 32;
+    if (ksplit < 1) ksplit = 1;
+    while ((uint64_t)((ksteps + ksplit - 1) / ksplit) * 32 >= (1ull << 24)) ksplit++;
+    return ksplit;
+}
+
 }  // namespace
 
 extern "C" {
@@ -279,7 +300,7 @@ int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t s
         hipEventCreate(&c->t0);
         hipEventRecord(c->t0, c->stream);
     }
-    // weighted-marginal work units: (row i, group cp of 64 partner sites)
+    // weighted-marginal work units: (row i, group cp of 32 partner sites)
     // with the group holding some m > i; each unit's sequence range is split
     // in wm_split parts, chosen to fill the last wave of work-groups best
     std::vector<uint32_t> units;
@@ -328,6 +349,7 @@ int mi3_destroy(mi3_ctx* c) {
     hipFree(c->red);
     hipFree(c->red_out);
     hipFree(c->wq);
+    hipFree(c->wacc);
     hipFree(c->wbad);
     hipFree(c->wm_units);
     hipFree(c->seq1);
@@ -575,15 +597,16 @@ int mi3_init_mcmc(mi3_ctx* c, uint32_t nsteps, uint64_t rng_offset, uint64_t rng
     if (!c) return MI3_EINVAL;
     if (c->mcmc) return fail(c, MI3_ESTATE, "Already initialized MCMC");
     if (nsteps == 0) return fail(c, MI3_EINVAL, "nsteps must be positive");
-    c->mcmc = true;
-    c->nsteps = nsteps;
-    c->walker_base = rng_span ? (uint32_t)(rng_offset / rng_span) * c->N : 0u;
     if (c->pos_cap < nsteps) {
         hipFree(c->pos);
         c->pos = nullptr;
+        c->pos_cap = 0;
         HIPCHK(c, hipMalloc(&c->pos, (size_t)nsteps * 4));
         c->pos_cap = nsteps;
     }
+    c->nsteps = nsteps;
+    c->walker_base = rng_span ? (uint32_t)(rng_offset / rng_span) * c->N : 0u;
+    c->mcmc = true;
     return MI3_OK;
 }
 
@@ -694,16 +717,7 @@ int mi3_calc_bicounts(mi3_ctx* c, const char* seqbuf, uint32_t nseq) {
     const uint32_t Lq = c->L * c->q;
     const uint32_t nblk = (Lq + 127) / 128;
     const uint32_t nbw = nblk * (nblk + 1) / 2;
-    // split the sequences so that about four waves per SIMD (one wave fits
-    // per SIMD at a time) share the work evenly, keeping at least 64 k-steps
-    // of 32 sequences per wave
-    const uint32_t ksteps = (nseq + 31) / 32;
-    const uint32_t target = 4 * 4 * c->ncu;
-    uint32_t ksplit = (target + nbw / 2) / nbw;
-    const uint32_t kmax = ksteps / 64 > 1 ? ksteps / 64 : 1;
-    if (ksplit > kmax) ksplit = kmax;
-    if (ksplit > 32) ksplit = 32;
-    if (ksplit < 1) ksplit = 1;
+    const uint32_t ksplit = bicount_split(c, nseq);
     Scope s(c, "calcBicounts", (size_t)nseq * c->L);
     if (ksplit > 1) {
         int frc = fill_u32_dev(c, cb->d, 0u, (uint64_t)c->nP * c->q * c->q);
@@ -717,6 +731,12 @@ int mi3_calc_bicounts(mi3_ctx* c, const char* seqbuf, uint32_t nseq) {
                        (uint32_t*)cb->d);
     HIPCHK(c, hipGetLastError());
     cb->version++;
+    return MI3_OK;
+}
+
+int mi3_bicount_split(mi3_ctx* c, uint32_t nseq, uint32_t* ksplit) {
+    if (!c || !ksplit) return MI3_EINVAL;
+    *ksplit = bicount_split(c, nseq);
     return MI3_OK;
 }
 
@@ -749,13 +769,16 @@ int mi3_weighted_marg(mi3_ctx* c, const char* seqbuf, const char* wbuf, uint32_t
     size_t qq = (size_t)c->q * c->q;
     rc = need(c, "bi", c->nP * qq * 4, &bb);
     if (rc) return rc;
-    if (c->wq_cap < nseq) {
+    const uint64_t nbins = (uint64_t)c->nP * qq;
+    if (c->wq_cap < nseq || !c->wq) {
         hipFree(c->wq);
         c->wq = nullptr;
-        HIPCHK(c, hipMalloc(&c->wq, (size_t)nseq * sizeof(uint32_t) + 256));
+        c->wq_cap = 0;
+        HIPCHK(c, hipMalloc(&c->wq, (size_t)nseq * sizeof(uint64_t) + 256));
         c->wq_cap = nseq;
     }
-    size_t lds = qq * WM_COLS * 4;
+    if (!c->wacc) HIPCHK(c, hipMalloc(&c->wacc, nbins * sizeof(unsigned long long)));
+    size_t lds = qq * WM_COLS * sizeof(unsigned long long);
     HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_weighted_marg),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     Scope s(c, "weightedMarg", (size_t)nseq * (c->L + 8));
@@ -768,16 +791,15 @@ int mi3_weighted_marg(mi3_ctx* c, const char* seqbuf, const char* wbuf, uint32_t
                        nb, 1, c->red_out);
     hipLaunchKernelGGL(k_weights_fixed, dim3(grid1(nseq ? nseq : 1)), dim3(256), 0, c->stream,
                        (const float*)wb->d, nseq, (const float*)c->red_out, c->wq, c->wbad);
-    const uint64_t nbins = (uint64_t)c->nP * qq;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid1(nbins)), dim3(256), 0, c->stream, (uint32_t*)bb->d,
-                       0u, nbins);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid1(2 * nbins)), dim3(256), 0, c->stream,
+                       (uint32_t*)c->wacc, 0u, 2 * nbins);
     hipLaunchKernelGGL(k_weighted_marg, dim3(c->n_units * c->wm_split), dim3(WM_THREADS), lds,
                        c->stream, (const uint8_t*)sb->d, (uint32_t)ld, nseq,
-                       (const uint32_t*)c->wq, c->L, c->q, (const uint32_t*)c->wm_units,
-                       c->wm_split, (uint32_t*)bb->d);
+                       (const uint64_t*)c->wq, c->L, c->q, (const uint32_t*)c->wm_units,
+                       c->wm_split, c->wacc);
     hipLaunchKernelGGL(k_fixed_to_float, dim3(grid1(nbins)), dim3(256), 0, c->stream,
-                       (uint32_t*)bb->d, nbins, (const float*)c->red_out,
-                       (const uint32_t*)c->wbad);
+                       (const unsigned long long*)c->wacc, (float*)bb->d, nbins,
+                       (const float*)c->red_out, (const uint32_t*)c->wbad);
     HIPCHK(c, hipGetLastError());
     bb->version++;
     return MI3_OK;

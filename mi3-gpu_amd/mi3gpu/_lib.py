@@ -49,6 +49,7 @@ SIGNATURES = {
     'mi3_gen_indep': (_i, [_P, _S]),
     'mi3_calc_energies': (_i, [_P, _S, _S, _S, _u32]),
     'mi3_calc_bicounts': (_i, [_P, _S, _u32]),
+    'mi3_bicount_split': (_i, [_P, _u32, _c.POINTER(_u32)]),
     'mi3_bicounts_to_bimarg': (_i, [_P, _u32]),
     'mi3_weighted_marg': (_i, [_P, _S, _S, _u32]),
     'mi3_renormalize_bimarg': (_i, [_P]),

@@ -194,25 +194,37 @@ int orc_energies(const float *J, uint32_t L, uint32_t q, const uint8_t *seqs, ui
     return 0;
 }
 
+/* site-major copy (L rows of N residues), so that each pair's scan reads two
+ * contiguous rows */
+static uint8_t *site_major(const uint8_t *seqs, uint32_t N, uint32_t L) {
+    uint8_t *T = (uint8_t *)malloc((size_t)N * L + 1);
+    if (!T) return NULL;
+    for (uint32_t w = 0; w < N; w++)
+        for (uint32_t i = 0; i < L; i++) T[(size_t)i * N + w] = seqs[(size_t)w * L + i];
+    return T;
+}
+
 int orc_bicounts(const uint8_t *seqs, uint32_t N, uint32_t L, uint32_t q, uint32_t *counts,
                  int nthreads) {
     size_t qq = (size_t)q * q;
     size_t nP = (size_t)L * (L - 1) / 2;
     memset(counts, 0, nP * qq * sizeof(uint32_t));
+    uint8_t *T = site_major(seqs, N, L);
+    if (!T) return -1;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
     for (long i = 0; i < (long)L - 1; i++) {
         size_t n0 = (size_t)i * (2 * L - i - 1) / 2;
+        const uint8_t *si = T + (size_t)i * N;
         for (uint32_t j = (uint32_t)i + 1; j < L; j++) {
             uint32_t *c = counts + (n0 + (j - i - 1)) * qq;
-            for (uint32_t w = 0; w < N; w++) {
-                const uint8_t *s = seqs + (size_t)w * L;
-                c[q * s[i] + s[j]]++;
-            }
+            const uint8_t *sj = T + (size_t)j * N;
+            for (uint32_t w = 0; w < N; w++) c[q * si[w] + sj[w]]++;
         }
     }
+    free(T);
     return 0;
 }
 
@@ -221,20 +233,22 @@ int orc_weighted_marg(const uint8_t *seqs, const float *wts, uint32_t N, uint32_
     size_t qq = (size_t)q * q;
     size_t nP = (size_t)L * (L - 1) / 2;
     memset(F, 0, nP * qq * sizeof(double));
+    uint8_t *T = site_major(seqs, N, L);
+    if (!T) return -1;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
     for (long i = 0; i < (long)L - 1; i++) {
         size_t n0 = (size_t)i * (2 * L - i - 1) / 2;
+        const uint8_t *si = T + (size_t)i * N;
         for (uint32_t j = (uint32_t)i + 1; j < L; j++) {
             double *c = F + (n0 + (j - i - 1)) * qq;
-            for (uint32_t w = 0; w < N; w++) {
-                const uint8_t *s = seqs + (size_t)w * L;
-                c[q * s[i] + s[j]] += (double)wts[w];
-            }
+            const uint8_t *sj = T + (size_t)j * N;
+            for (uint32_t w = 0; w < N; w++) c[q * si[w] + sj[w]] += (double)wts[w];
         }
     }
+    free(T);
     return 0;
 }
 

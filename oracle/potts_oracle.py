@@ -424,10 +424,16 @@ def renormalize(bi):
 
 
 def iter_newton(seqs, J, bimarg_target, gamma, pc, newton_steps, frac_neff,
-                reg=None, reg_lambda=0.0, energies_fn=None, wmarg_fn=None):
+                reg=None, reg_lambda=0.0, energies_fn=None, wmarg_fn=None, beta=None):
     """The Zwanzig-reweighting quasi-Newton loop of one round on one GPU
     (NewtonSteps.py:159-268 with merge_bimarg = renormalize on one device,
     node_manager.py:283-287).
+
+    With `beta`, the sequences were sampled at inverse temperature beta and
+    are reweighted to beta = 1 (NewtonSteps.py:206-214, 223-224): every
+    iteration's energies gain -(beta-1)*E(s) under the starting couplings,
+    the first reference energy is the minimum of that term, and the stop
+    rule compares Neff with the Neff of the reweighting alone.
 
     Returns (iterations i, trialJ = J + dJ, predicted bimarg, weights,
     Neff history).  `energies_fn` / `wmarg_fn` can substitute a faster
@@ -441,6 +447,12 @@ def iter_newton(seqs, J, bimarg_target, gamma, pc, newton_steps, frac_neff,
     dJ = np.zeros_like(np.asarray(J, dtype=np.float64))
     N0 = N
     ref_dE = 0.0
+    beta_mod = None
+    if beta is not None:
+        beta_mod = -(beta - 1) * np.asarray(efn(seqs, J), dtype=np.float64)
+        ref_dE = float(np.min(beta_mod))
+        w0 = np.exp(-(beta_mod - ref_dE))
+        N0 = np.sum(w0) ** 2 / np.sum(w0 ** 2)
     neffs = []
     w = None
     i = 0
@@ -451,6 +463,8 @@ def iter_newton(seqs, J, bimarg_target, gamma, pc, newton_steps, frac_neff,
         elif reg == 'l2z':
             dJ, _ = reg_l2z(bi, gamma, pc, reg_lambda, J, dJ)
         dE = efn(seqs, dJ)
+        if beta_mod is not None:
+            dE = dE + beta_mod
         mindE = min_floats(dE)
         w = dE_to_weights(dE, ref_dE)
         s1, s2 = weight_stats(w)

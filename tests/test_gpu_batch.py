@@ -9,9 +9,10 @@ from oracle import cboracle
 pytestmark = pytest.mark.gpu
 
 # (L, q, walkers): register classes of one to five state words (L <= 128, <= 256,
-# <= 384, > 512) x q classes
+# <= 384, <= 512, > 512) x q classes; the four-word models cover the top of
+# the batched workload's L range (50..400)
 MODELS = [(20, 4, 512), (70, 21, 1024), (140, 4, 512), (33, 21, 512), (260, 4, 512),
-          (99, 21, 512), (600, 21, 512)]
+          (99, 21, 512), (600, 21, 512), (400, 21, 1024), (390, 4, 512)]
 
 
 def _setup(seed=40):

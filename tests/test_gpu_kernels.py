@@ -218,18 +218,18 @@ def test_weighted_marg_and_renorm(gpu_ok, tmp_path, L, q, N):
     g.weightedMarg('main')
     F = g.getBuf('bi').read()
     Fx = cboracle.weighted_marg(s, w, q)
-    np.testing.assert_allclose(F, Fx, rtol=1e-5, atol=1e-6 * w.sum())
+    occ = Fx > 0
+    # exact 64-bit fixed-point sums rounded once to float: within an ulp
+    np.testing.assert_allclose(F[occ], Fx[occ], rtol=2.4e-7, atol=0)
+    assert np.all(F[~occ] == 0)
     g.renormalize_bimarg()
     np.testing.assert_allclose(g.getBuf('bi').read(), po.renormalize(Fx), rtol=1e-5, atol=1e-7)
-    # unit weights reduce to the counts (exactly when N is a power of two: each
-    # weight is then exactly 2^31 / N in the fixed-point form)
+    # unit weights reduce to the counts exactly (the bin sums are c * 2^62/N
+    # to within 2^-53 relative, which rounds to the float of the count c)
     g.setBuf('weights', np.ones(N, '<f4'))
     g.weightedMarg('main')
     counts = cboracle.bicounts(s, q).astype('f4')
-    if N & (N - 1) == 0:
-        np.testing.assert_array_equal(g.getBuf('bi').read(), counts)
-    else:
-        np.testing.assert_allclose(g.getBuf('bi').read(), counts, rtol=1e-6)
+    np.testing.assert_array_equal(g.getBuf('bi').read(), counts)
 
 
 def test_weighted_marg_ragged_large(gpu_ok, tmp_path):
@@ -245,7 +245,9 @@ def test_weighted_marg_ragged_large(gpu_ok, tmp_path):
     g.weightedMarg('large')
     F = g.getBuf('bi').read()
     Fx = cboracle.weighted_marg(s, w, q)
-    np.testing.assert_allclose(F, Fx, rtol=1e-5, atol=1e-6 * w.sum())
+    occ = Fx > 0
+    np.testing.assert_allclose(F[occ], Fx[occ], rtol=2.4e-7, atol=0)
+    assert np.all(F[~occ] == 0)
 
 
 def test_min_weights_stats(gpu_ok, tmp_path):
@@ -298,8 +300,13 @@ def test_iter_newton_matches_oracle(gpu_ok, tmp_path, reg):
         wmarg_fn=lambda ss, w: cboracle.weighted_marg(ss, w.astype('f4'), q))
     print("iterations", i, oi, "neff", neffs[-1])
     assert i == oi
-    np.testing.assert_allclose(trialJ, oJ, rtol=1e-4, atol=1e-6)
-    np.testing.assert_allclose(bim, obim, rtol=1e-4, atol=1e-7)
+    dJ_scale = np.abs(oJ - J).max()
+    err_J = np.abs(trialJ - oJ) / (np.abs(oJ) + dJ_scale)
+    occ = obim > 0
+    err_b = np.abs(bim - obim)[occ] / obim[occ]
+    print(f"J+dJ rel err {err_J.max():.3g}, bimarg rel err {err_b.max():.3g}")
+    assert err_J.max() < 1e-5
+    assert err_b.max() < 1e-5
 
 
 REG_CASES = [

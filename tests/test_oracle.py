@@ -342,6 +342,30 @@ def test_jeffreys_pseudocount():
     assert out.min() > b.min()
 
 
+def test_jeffreys_pseudocount_hand_example():
+    """One pair, q=2, N=1 written out by hand from pseudocount.py:8-24:
+    mu = q/(2N+q) = 1/2; f_i = (0.7, 0.3) (row sums), f_j = (0.6, 0.4)
+    (column sums); fifj = f_i(a) + f_j(b) = (1.3, 1.1, 0.9, 0.7);
+    (1-mu)^2 ff + (1-mu) mu fifj / q + (mu/q)^2
+      = (0.125, 0.05, 0.025, 0.05) + (0.1625, 0.1375, 0.1125, 0.0875) + 0.0625
+      = (0.35, 0.25, 0.2, 0.2), which already sums to 1.  A wrong mu (the
+    Bayes q/(N+q)) or a product f_i f_j in place of the outer sum gives other
+    values."""
+    ff = np.array([[0.5, 0.2, 0.1, 0.2]])
+    want = np.array([[0.35, 0.25, 0.2, 0.2]])
+    np.testing.assert_allclose(po.jeffreys_pseudocount(ff, 1), want, rtol=1e-14)
+    from mi3gpu.utils.pseudocount import mutation_pc
+    np.testing.assert_allclose(mutation_pc(ff, 1), want, rtol=1e-14)
+    # the Bayes form: mu = 2/3 -> (1/9) ff + (2/9) fifj/2 + 1/9, renormalised
+    bayes = ff / 9 + fifj_sum(ff) / 9 + 1 / 9
+    np.testing.assert_allclose(mutation_pc(ff, 1, mode='bayes'), bayes / bayes.sum(),
+                               rtol=1e-14)
+
+
+def fifj_sum(ff):
+    return np.array([[1.3, 1.1, 0.9, 0.7]])
+
+
 def test_Xijab_contraction_matches_quadruple_loop():
     """getXijab's literal q^4 loop (mcmc.cl:1052-1067) on random blocks."""
     rng = np.random.default_rng(21)
