@@ -1,4 +1,12 @@
 #!/usr/bin/env python3
+# This file keeps the interface and behaviour of the file of the same name
+# in Mi3-GPU, Copyright 2020 Allan Haldane, and is a derived work of it.
+# Mi3-GPU is free software: you can redistribute it and/or modify it under
+# the terms of the GNU General Public License as published by the Free
+# Software Foundation, version 3 of the License; this file is distributed
+# under the same licence, WITHOUT ANY WARRANTY; without even the implied
+# warranty of MERCHANTABILITY or FITNESS FOR A PARTICULAR PURPOSE.  See the
+# GNU General Public License <http://www.gnu.org/licenses/> for details.
 """Mi3 command line: `Mi3.py {infer,gen,energies,benchmark,subseq} [options]`
 (Mi3.py of the reference).
 
@@ -368,38 +376,45 @@ def updateLq(L, q, newL, newq, name):
     return L, q
 
 
+def _load_f4(fn, what):
+    arr = np.load(fn)
+    if arr.dtype != np.dtype('<f4'):
+        raise Exception(f"{what} must be in 'f4' format")
+    return arr
+
+
+GENERATED = ('uniform', 'independent')
+
+
 def getCouplings(args, L, q, unimarg, log):
-    """'uniform' / 'independent' / file / init_model dir (Mi3.py:1158-1203)."""
-    if args.couplings is None and args.init_model in ['uniform', 'independent']:
-        args.couplings = args.init_model
-    if args.couplings:
-        if args.couplings in ['uniform', 'independent'] and L is None:
+    """Initial couplings (Mi3.py:1158-1203).  --couplings wins over
+    --init_model; either may be 'uniform' (all zero) or 'independent' (the
+    fieldless form of fields -log f_i); otherwise --couplings is a .npy file
+    and --init_model a run directory holding J.npy."""
+    from_dir = not args.couplings
+    source = args.init_model if from_dir else args.couplings
+    if source is None:
+        raise Exception("didn't get couplings or init_model")
+    if source in GENERATED:
+        args.couplings = source
+        if L is None:
             raise Exception("Need L to generate couplings")
-        if args.couplings == 'uniform':
+        zero = np.zeros((L * (L - 1) // 2, q * q), dtype='<f4')
+        if source == 'uniform':
             log("Setting Initial couplings to uniform frequencies")
-            couplings = np.zeros((L * (L - 1) // 2, q * q), dtype='<f4')
-        elif args.couplings == 'independent':
+            couplings = zero
+        else:
             log("Setting Initial couplings to independent model")
             if unimarg is None:
                 raise Exception("Need univariate marginals to generate independent model "
                                 "couplings")
-            J = np.zeros((L * (L - 1) // 2, q * q), dtype='<f4')
-            couplings = fieldlessGaugeEven(-np.log(unimarg), J)[1].astype('<f4')
-        else:
-            log(f"Reading couplings from file {args.couplings}")
-            couplings = np.load(args.couplings)
-            if couplings.dtype != np.dtype('<f4'):
-                raise Exception("Couplings must be in 'f4' format")
-    elif args.init_model and args.init_model not in ['uniform', 'independent']:
-        fn = Path(args.init_model, 'J.npy')
-        if not fn.is_file():
+            couplings = fieldlessGaugeEven(-np.log(unimarg), zero)[1].astype('<f4')
+    else:
+        fn = Path(source, 'J.npy') if from_dir else source
+        if from_dir and not fn.is_file():
             raise Exception(f"could not find file {fn}")
         log(f"Reading couplings from file {fn}")
-        couplings = np.load(fn)
-        if couplings.dtype != np.dtype('<f4'):
-            raise Exception("Couplings must be in 'f4' format")
-    else:
-        raise Exception("didn't get couplings or init_model")
+        couplings = _load_f4(fn, "Couplings")
     L2, q2 = getLq(couplings)
     L, q = updateLq(L, q, L2, q2, 'couplings')
     return couplings, L, q
@@ -456,8 +471,33 @@ def loadSequenceDir(sdir, bufname, alpha, log):
     return loadSequenceFile(Path(sdir) / 'seqs', alpha, log)
 
 
+def _first_source(candidates):
+    """The first (kind, value) whose value is given and is a generator name
+    exactly when kind is 'gen'."""
+    for kind, v in candidates:
+        if v is not None and (kind == 'gen') == (v in GENERATED):
+            return kind, v
+    return None, None
+
+
+def _fit_rows(seqs, n, log):
+    """Repeat or truncate `seqs` to n rows."""
+    have = seqs.shape[0]
+    if n > have:
+        log(f"Repeating {have} sequences to make {n}")
+        return repeatseqs(seqs, n)
+    if n < have:
+        log(f"Truncating {have} sequences to make {n}")
+        return seqs[:n]
+    return seqs
+
+
 def process_sequence_args(args, L, alpha, log, unimarg=None, nseqs=None, needseed=False):
-    """Initial walker sequences or a seed sequence (Mi3.py:1205-1270)."""
+    """Initial walker sequences and/or a seed sequence (Mi3.py:1205-1270).
+    Walkers: a generator named by --seqs, then one named by --init_model,
+    then a --seqs file, then the `seqs` of an --init_model directory.  Seed:
+    a generator or letters/file given by --seedseq, then an --init_model
+    generator or the `seedseq` file of its directory."""
     if not nseqs and not needseed:
         return {}
     log("Sequence Setup")
@@ -465,41 +505,33 @@ def process_sequence_args(args, L, alpha, log, unimarg=None, nseqs=None, needsee
     q = len(alpha)
     seedseq, seqs = None, None
     if nseqs is not None:
-        if args.seqs in ['uniform', 'independent']:
-            seqs = generateSequences(args.seqs, L, q, nseqs, log, unimarg)
-        elif args.init_model in ['uniform', 'independent']:
-            seqs = generateSequences(args.init_model, L, q, nseqs, log, unimarg)
-        elif args.seqs is not None:
-            seqs = loadSequenceFile(args.seqs, alpha, log)
-        elif args.init_model is not None:
-            seqs = loadSequenceDir(args.init_model, '', alpha, log)
+        kind, v = _first_source([('gen', args.seqs), ('gen', args.init_model),
+                                 ('file', args.seqs), ('dir', args.init_model)])
+        if kind == 'gen':
+            seqs = generateSequences(v, L, q, nseqs, log, unimarg)
+        elif kind == 'file':
+            seqs = loadSequenceFile(v, alpha, log)
+        elif kind == 'dir':
+            seqs = loadSequenceDir(v, '', alpha, log)
         if seqs is None:
             raise Exception(f"Did not find requested {nseqs} sequences")
-        n = seqs.shape[0]
-        if nseqs > n:
-            log(f"Repeating {n} sequences to make {nseqs}")
-            seqs = repeatseqs(seqs, nseqs)
-        elif nseqs < n:
-            log(f"Truncating {n} sequences to make {nseqs}")
-            seqs = seqs[:nseqs]
+        seqs = _fit_rows(seqs, nseqs, log)
     if needseed:
-        if args.seedseq in ['uniform', 'independent']:
-            seedseq = generateSequences(args.seedseq, L, q, 1, log, unimarg)[0]
-            origin = args.seedseq
-        elif args.seedseq is not None:
-            try:
-                seedseq = np.array([alpha.index(c) for c in args.seedseq], dtype='<u1')
+        kind, v = _first_source([('gen', args.seedseq), ('given', args.seedseq),
+                                 ('gen', args.init_model), ('dir', args.init_model)])
+        origin = v if kind == 'gen' else 'from file'
+        if kind == 'gen':
+            seedseq = generateSequences(v, L, q, 1, log, unimarg)[0]
+        elif kind == 'given':
+            if all(c in alpha for c in v):
+                seedseq = np.array([alpha.index(c) for c in v], dtype='<u1')
                 origin = 'supplied'
-            except ValueError:
-                seedseq = loadseedseq(args.seedseq, alpha, log)
-                origin = 'from file'
-        elif args.init_model in ['uniform', 'independent']:
-            seedseq = generateSequences(args.init_model, L, q, 1, log, unimarg)[0]
-            origin = args.init_model
-        elif args.init_model is not None:
-            seedseq = loadseedseq(Path(args.init_model, 'seedseq'), alpha, log)
-            origin = 'from file'
-        log(f"Seed seq ({origin}): {''.join(alpha[x] for x in seedseq)}")
+            else:
+                seedseq = loadseedseq(v, alpha, log)
+        elif kind == 'dir':
+            seedseq = loadseedseq(Path(v, 'seedseq'), alpha, log)
+        if seedseq is not None:
+            log(f"Seed seq ({origin}): {''.join(alpha[x] for x in seedseq)}")
     log("")
     return attrdict({'seedseq': seedseq, 'seqs': seqs})
 
@@ -801,6 +833,16 @@ def equilibrate(orig_args, args, log):
     return bimarg, energies
 
 
+def _pad_rows(seqs, multiple):
+    """`seqs` with zero rows appended up to a positive multiple of
+    `multiple` rows (the walker count the device needs)."""
+    n = seqs.shape[0]
+    want = max(multiple, -(-n // multiple) * multiple)
+    if want == n:
+        return seqs
+    return np.concatenate([seqs, np.zeros((want - n, seqs.shape[1]), dtype=seqs.dtype)])
+
+
 def subseqFreq(orig_args, args, log):
     """`subseq`: log relative frequency of subsequences at fixed positions
     against a background sample (Mi3.py:872-976).  For every subsequence
@@ -832,61 +874,66 @@ def subseqFreq(orig_args, args, log):
     p.update(process_potts_args(args, getattr(args, 'L', None), None, None, log))
     L, q, alpha = p.L, p.q, p.alpha
 
-    bseqs = loadSequenceFile(args.backgroundseqs, alpha, log)
-    sseqs = loadSequenceFile(args.subseqs, alpha, log)
-    large, small = (sseqs, bseqs) if args.iterbackgrounds else (bseqs, sseqs)
-    ns = len(small)
-    if ns < 512 or ns % 512 != 0:
-        n_new = ns - 1 + 512 - ((ns - 1) % 512)
-        small = np.concatenate([small, np.zeros((n_new - ns, L), dtype='u1')], axis=0)
-    args.nwalkers = small.shape[0]
+    background = loadSequenceFile(args.backgroundseqs, alpha, log)
+    subseqs = loadSequenceFile(args.subseqs, alpha, log)
+    positions = np.array([int(x) for x in args.fixpos.split(',')])
+    # One set stays in the large device buffer; the members of the other
+    # (the walker buffer) are substituted into it one at a time.  Normally
+    # the subsequences substitute their fixed positions into the background;
+    # with --iterbackgrounds each background sequence substitutes everything
+    # but those positions into the subsequence set.
+    if args.iterbackgrounds:
+        walkers, stored = background, subseqs
+        substituted = np.setdiff1d(np.arange(L), positions)
+    else:
+        walkers, stored = subseqs, background
+        substituted = positions
+    nwalk = len(walkers)
+    walkers = _pad_rows(walkers, 512)
+    args.nwalkers = walkers.shape[0]
     p.update(process_GPU_args(args, L, q, p.outdir, log))
     p['rngseed'] = 0
     gpus = setup_GPUs(p, log, splitwalkers=False)
     gpus.initMCMC(p.nsteps)
-    gpulseqs = np.array_split(large, gpus.ngpus)
-    gpus.initLargeBufs(gpulseqs[0].shape[0])
+    stored_parts = np.array_split(stored, gpus.ngpus)
+    gpus.initLargeBufs(stored_parts[0].shape[0])
     gpus.initSubseq()
+    log(substituted)
+    marks = np.zeros(L, dtype='u1')
+    marks[substituted] = 1
 
-    fixedpos = np.array([int(x) for x in args.fixpos.split(',')])
-    if args.iterbackgrounds:
-        members = set(fixedpos)
-        fixedpos = np.array([i for i in range(L) if i not in members])
-    log(fixedpos)
-    fixedmarks = np.zeros(L, dtype='u1')
-    fixedmarks[fixedpos] = 1
-
-    gpus.setBuf('seq main', small)
-    gpus.setBuf('seq large', gpulseqs)
-    gpus.markPos(fixedmarks)
+    gpus.setBuf('seq main', walkers)
+    gpus.setBuf('seq large', stored_parts)
+    gpus.markPos(marks)
     gpus.setBuf('J', p.couplings)
     log("")
     log("Subsequence Frequency Calculation")
     log("=================================")
     log("")
+
+    def substituted_energies(n):
+        """Energies of the stored set with walker n's marked residues."""
+        gpus.copySubseq(n)
+        gpus.calcEnergies('large')
+        return gpus.collect('E large')
+
     gpus.calcEnergies('main')
-    origEs = gpus.collect('E main')[:ns]
+    walkerE = gpus.collect('E main')[:nwalk]
     log("Getting substituted energies...")
     if args.iterbackgrounds:
-        logf = np.full(len(large), -np.inf)
-        for n, eo in enumerate(origEs):
-            gpus.copySubseq(n)
-            gpus.calcEnergies('large')
-            energies = gpus.collect('E large')
-            np.logaddexp(logf, eo - energies, out=logf)
+        # log f of stored sequence k: logsumexp over background walkers n of
+        # E_n - E(k with n's residues outside the fixed positions)
+        logf = np.full(len(stored), -np.inf)
+        for n in range(nwalk):
+            logf = np.logaddexp(logf, walkerE[n] - substituted_energies(n))
     else:
-        # per subsequence n: log sum_b exp(E_b - E_b[n's fixed positions]) over
-        # the background sequences b.  (The reference subtracts the
+        # log f of subsequence n: logsumexp over background sequences b of
+        # E_b - E(b with n's fixed positions).  (The reference subtracts the
         # subsequence set's energies here, which only broadcasts when both
         # sets have the same size; the background energies are meant.)
         gpus.calcEnergies('large')
-        backEs = gpus.collect('E large')
-        logf = np.zeros(ns)
-        for n in range(ns):
-            gpus.copySubseq(n)
-            gpus.calcEnergies('large')
-            energies = gpus.collect('E large')
-            logf[n] = logsumexp(backEs - energies)
+        storedE = gpus.collect('E large')
+        logf = np.array([logsumexp(storedE - substituted_energies(n)) for n in range(nwalk)])
     log("Done! Saving result (log frequency) to file.")
     np.save(p.outdir / 'probs.npy', logf)
     logfile.close()

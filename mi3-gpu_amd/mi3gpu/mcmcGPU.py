@@ -1,3 +1,11 @@
+# This file keeps the interface and behaviour of the file of the same name
+# in Mi3-GPU, Copyright 2020 Allan Haldane, and is a derived work of it.
+# Mi3-GPU is free software: you can redistribute it and/or modify it under
+# the terms of the GNU General Public License as published by the Free
+# Software Foundation, version 3 of the License; this file is distributed
+# under the same licence, WITHOUT ANY WARRANTY; without even the implied
+# warranty of MERCHANTABILITY or FITNESS FOR A PARTICULAR PURPOSE.  See the
+# GNU General Public License <http://www.gnu.org/licenses/> for details.
 """Per-device host wrapper of the MI355X Mi3 engine.
 
 Drop-in for the reference's mcmcGPU module (mi3gpu/mcmcGPU.py): same class

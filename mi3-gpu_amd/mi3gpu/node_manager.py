@@ -1,3 +1,11 @@
+# This file keeps the interface and behaviour of the file of the same name
+# in Mi3-GPU, Copyright 2020 Allan Haldane, and is a derived work of it.
+# Mi3-GPU is free software: you can redistribute it and/or modify it under
+# the terms of the GNU General Public License as published by the Free
+# Software Foundation, version 3 of the License; this file is distributed
+# under the same licence, WITHOUT ANY WARRANTY; without even the implied
+# warranty of MERCHANTABILITY or FITNESS FOR A PARTICULAR PURPOSE.  See the
+# GNU General Public License <http://www.gnu.org/licenses/> for details.
 """Device-group interface of the inference driver (node_manager.py of the
 reference).  `GPU_node` keeps the reference's method surface, which is what
 NewtonSteps and Mi3 call; on this build a node holds the devices of one host
@@ -7,59 +15,67 @@ configuration) merge_bimarg reduces to renormalisation
 import numpy as np
 
 
-def sumarr(arrlist):
-    tot = arrlist[0].copy()
-    for a in arrlist[1:]:
-        np.add(tot, a, tot)
-    return tot
+# How the per-device copies of a buffer become one host array, keyed by the
+# first word of the buffer name (node_manager.py:168-196): per-walker
+# buffers are stacked in device order, counts add up and bimarginals are
+# averaged.  A name not listed here comes back as the list of device arrays.
+def _stacked(parts):
+    return np.concatenate(parts)
 
 
-def meanarr(arrlist):
-    return sumarr(arrlist) / len(arrlist)
+def _added(parts):
+    out = np.array(parts[0], copy=True)
+    for p in parts[1:]:
+        out += p
+    return out
 
 
-def skip_single(f):
-    def skipper(x):
-        return x[0] if len(x) == 1 else f(x)
-    return skipper
+def _averaged(parts):
+    return _added(parts) / len(parts)
+
+
+_COMBINE = {'seq': _stacked, 'E': _stacked, 'weights': _stacked, 'Bs': _stacked,
+            'bicount': _added, 'bi': _averaged}
+
+
+def combine(bufname, parts):
+    """One host array from the per-device arrays `parts` of `bufname`."""
+    fn = _COMBINE.get(bufname.split()[0])
+    if fn is None:
+        return parts
+    return parts[0] if len(parts) == 1 else fn(parts)
 
 
 class GPU_node:
-    groupfunc = {
-        'weights': skip_single(np.concatenate),
-        'E': skip_single(np.concatenate),
-        'Bs': skip_single(np.concatenate),
-        'bicount': skip_single(sumarr),
-        'bi': skip_single(meanarr),
-        'seq': skip_single(np.concatenate),
-    }
+    """The devices of one host behind one interface; every method runs on
+    each device in order."""
 
     def __init__(self, gpus):
-        self.gpus = gpus
+        self.gpus = list(gpus)
         self.head_node = self
 
     @property
     def head_gpu(self):
-        return type(self)([self.gpus[0]])
-
-    @property
-    def nwalkers(self):
-        return sum(g.nwalkers for g in self.gpus)
-
-    @property
-    def nseq(self):
-        ret = {'main': sum(g.nseq['main'] for g in self.gpus)}
-        if 'large' in self.gpus[0].nseq:
-            ret['large'] = sum(g.nseq.get('large', 0) for g in self.gpus)
-        return ret
+        """A node made of the first device alone."""
+        return GPU_node(self.gpus[:1])
 
     @property
     def ngpus(self):
         return len(self.gpus)
 
     @property
+    def nwalkers(self):
+        return int(np.sum([g.nwalkers for g in self.gpus]))
+
+    @property
+    def nseq(self):
+        """Sequences per buffer kind, over all devices."""
+        kinds = ['main'] + (['large'] if 'large' in self.gpus[0].nseq else [])
+        return {k: int(np.sum([g.nseq.get(k, 0) for g in self.gpus])) for k in kinds}
+
+    @property
     def gpu_list(self):
-        return ["({}) device {}".format(g.gpunum, g.device) for g in self.gpus]
+        return [f"({g.gpunum}) device {g.device}" for g in self.gpus]
 
     def _each(self, meth, *args):
         for g in self.gpus:
@@ -143,10 +159,8 @@ class GPU_node:
         """Read buffers from every device and combine them per buffer type
         (node_manager.py:168-196)."""
         if isinstance(bufs, str):
-            f = self.groupfunc.get(bufs.split()[0], lambda x: x)
-            return f(self.readBufs(bufs))
-        return [self.groupfunc.get(bn.split()[0], lambda x: x)(arrs)
-                for bn, arrs in zip(bufs, self.readBufs(bufs))]
+            return combine(bufs, self.readBufs(bufs))
+        return [combine(bn, arrs) for bn, arrs in zip(bufs, self.readBufs(bufs))]
 
     def setBuf(self, bufname, dat):
         if isinstance(dat, list):
@@ -198,7 +212,7 @@ class GPU_node:
         if len(self.gpus) == 1:
             self.gpus[0].renormalize_bimarg()
             return
-        bi = sumarr([g.getBuf('bi').read() for g in self.gpus])
+        bi = _added([g.getBuf('bi').read() for g in self.gpus])
         bi = (bi / bi.sum(axis=1, keepdims=True)).astype('<f4')
         for g in self.gpus:
             g.setBuf('bi', bi)
