@@ -54,6 +54,21 @@ HBM_PEAK_GBS = 8000.0
 # (L, q, walkers, sweeps per launch, walkers per work-group).
 MEASURED_TRAFFIC = {(232, 21, 2 ** 18, 8, 512): 13.41e9}
 
+# VALU issue of the same k_sweep launch from SQ counters
+# (SQ_ACTIVE_INST_VALU per CU over SQ_BUSY_CYCLES; summary and command in
+# profiles/k_sweep_L232_q21_N262144_kernel_trace.md).  Same key.
+MEASURED_VALU_BUSY = {(232, 21, 2 ** 18, 8, 512): 0.71}
+
+
+def host_cores():
+    """Cores this job may use: its CPU affinity, capped by OMP_NUM_THREADS
+    when the host sets it (the share a shared GPU host gives one job)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else os.cpu_count()
+    omp = os.environ.get('OMP_NUM_THREADS')
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, int(n or 1))
+
 
 def production_model(L=232, q=21, seed=1, hs=1.0, js=0.05):
     """h ~ N(0,hs), J ~ N(0,js), zero-sum gauge, folded to fieldless float32."""
@@ -65,10 +80,16 @@ def production_model(L=232, q=21, seed=1, hs=1.0, js=0.05):
     return fieldlessGaugeEven(h, J)[1].astype('<f4')
 
 
+def _pf_class(q):
+    """float4 loads per thread per staged coupling chunk (launch.h pf_class)."""
+    need = (q * q * 32 // 4 + 511) // 512
+    return 1 if need <= 1 else 4 if need <= 4 else 7 if need <= 7 else 10
+
+
 def cpu_baseline(J, L, q, seed, target_s=20.0):
     """Time the CPU port of the sampler on this host's cores."""
     from oracle import cboracle
-    nthr = int(min(16, os.cpu_count() or 1))
+    nthr = host_cores()
     walkers = 64 * nthr
     rng = np.random.default_rng(seed)
     s = rng.integers(0, q, size=(walkers, L)).astype('<u1')
@@ -138,14 +159,17 @@ def measure_ess(J, seqs, L, q, T, seed, wgsize):
             "mean_E_second_half": float(E[half:].mean())}
 
 
-def measure_rounds(target, L, q, N, rounds, seed):
-    """`Mi3.py infer` rounds at the production size; per-round wall times."""
+def measure_rounds(target, L, q, N, rounds, seed, equiltime='8', nsteps=None):
+    """`Mi3.py infer` rounds at the production size; per-round wall times.
+    Default schedule: 8 launches of 16 sweeps (128 sweeps per round);
+    equiltime 'auto' uses the reference's equilibration test instead."""
     import mi3gpu.Mi3 as Mi3
     with tempfile.TemporaryDirectory() as d:
         np.save(Path(d) / 'target.npy', target)
         args = ['infer', '--init_model', 'independent', '--bimarg', str(Path(d) / 'target.npy'),
-                '--alpha=' + ALPHA21[:q], '--nwalkers', str(N), '--nsteps', str(16 * L),
-                '--equiltime', '8', '--mcsteps', str(rounds), '--rngseed', str(seed),
+                '--alpha=' + ALPHA21[:q], '--nwalkers', str(N),
+                '--nsteps', str(nsteps or 16 * L), '--equiltime', str(equiltime),
+                '--mcsteps', str(rounds), '--rngseed', str(seed),
                 '--outdir', str(Path(d) / 'infer')]
         import contextlib
         # Mi3's own log lines go to stderr; stdout keeps the one JSON line
@@ -166,6 +190,59 @@ def measure_rounds(target, L, q, N, rounds, seed):
                 "cumulative_s": float(np.sum(wall[:at + 1]))},
             "note": "rounds from the independent-site model; SSR per round; fixed_ssr is the "
                     "first round with SSR <= 2 x floor (null if not reached)"}
+
+
+# The reference's published end-to-end result (UserGuide.md:103-113): the
+# HIV protease run of examples/HIV_inference/pbs.sh:11-20, rounds run_04..06
+# and run_63 (Ferr is the mean relative error over target bimarginals
+# above 0.01, the log's rel% / 100).
+HIV_PUBLISHED = {4: {'jstep': 4092, 'ssr': 0.206, 'ferr': 0.00992, 'X': -119.709, 'Xo': -122.201},
+                 5: {'jstep': 5115, 'ssr': 0.204, 'ferr': 0.00984, 'X': -120.186, 'Xo': -122.374},
+                 6: {'jstep': 6138, 'ssr': 0.203, 'ferr': 0.00971, 'X': -120.275, 'Xo': -122.534},
+                 63: {'ssr': 0.010, 'ferr': 0.00856, 'X': -10.118, 'Xo': -10.878}}
+
+
+def measure_hiv(rounds, walkers=2 ** 18, seed=7):
+    """`Mi3.py infer` with the settings of the reference's HIV example
+    (examples/HIV_inference/pbs.sh:11-20: independent start, 2^18 walkers,
+    damping 0.01, l1z 1e-4, 64 rounds; reference defaults otherwise --
+    nsteps 2048, equiltime auto, newtonsteps 1024, gamma 4e-4) on the
+    reference's target (L=93, q=4).  Per-round SSR, Ferr, X and wall time
+    beside the published values, and the first round within twice the SSR
+    floor sum f(1-f)/N."""
+    import contextlib
+    import mi3gpu.Mi3 as Mi3
+    target = np.load(REPO / 'tests' / 'golden' / 'hiv_bimarg_pc.npy')
+    with tempfile.TemporaryDirectory() as d:
+        args = ['infer', '--init_model', 'independent',
+                '--bimarg', str(REPO / 'tests' / 'golden' / 'hiv_bimarg_pc.npy'),
+                '--alpha', 'ABCD', '--nwalkers', str(walkers), '--damping', '0.01',
+                '--mcsteps', str(rounds), '--reg', 'l1z:0.0001', '--rngseed', str(seed),
+                '--outdir', str(Path(d) / 'hiv')]
+        t0 = time.perf_counter()
+        with contextlib.redirect_stdout(sys.stderr):
+            p = Mi3.main(args)
+        total = time.perf_counter() - t0
+    st = p.round_stats
+    floor = float(np.sum(target * (1 - target)) / walkers)
+    at = next((k for k, r in enumerate(st) if r['ssr'] <= 2 * floor), None)
+    keep = ('ssr', 'ferr', 'X', 'Xo', 'jstep', 'newton_steps', 'equil_launches', 'wall_s',
+            'mcmc_s', 'newton_s', 'status_write_s')
+    per_round = [{k: r[k] for k in keep} for r in st]
+    compare = {f"run_{k:02d}": {"published": v, "measured": per_round[k]}
+               for k, v in HIV_PUBLISHED.items() if k < len(per_round)}
+    wall = [r['wall_s'] for r in st]
+    return {"workload": "Mi3.py infer, examples/HIV_inference/pbs.sh settings, L=93 q=4, "
+                        f"{walkers} walkers, {rounds} rounds, reference defaults",
+            "rounds": per_round, "published_vs_measured": compare, "ssr_floor": floor,
+            "s_per_round": float(np.mean(wall)),
+            "s_per_round_without_seq_write": float(np.mean(
+                [r['wall_s'] - r['status_write_s'] for r in st])),
+            "total_s": total,
+            "fixed_ssr": None if at is None else {
+                "round": at, "ssr": st[at]['ssr'], "s": st[at]['wall_s'],
+                "s_without_seq_write": st[at]['wall_s'] - st[at]['status_write_s'],
+                "cumulative_s": float(np.sum(wall[:at + 1]))}}
 
 
 def measure_batch(rounds, sweeps=64, nmodels=64, seed=4):
@@ -403,7 +480,7 @@ def measure_config1(seed=0):
             p = Mi3.main(args)
         gpu = p.round_stats[0]
         # CPU parts on the same sizes
-        nthr = int(min(16, os.cpu_count() or 1))
+        nthr = host_cores()
         s0 = np.random.default_rng(1).integers(0, q, (N, L)).astype('u1')
         t0 = time.perf_counter()
         s1 = cboracle.metropolis_f32(s0, J, 7, 0, 64 * L, nthreads=nthr)
@@ -412,7 +489,7 @@ def measure_config1(seed=0):
         bc = po.bicounts(s1, q)
         t_bc = time.perf_counter() - t0
         t0 = time.perf_counter()
-        it, *_ = po.iter_newton(s1, J, target, 4e-4, 1e-3, 2, 0.0)
+        it, *_ = po.iter_newton(s1, J, target, 4e-4, 1e-3, 4, 0.0)
         t_it = (time.perf_counter() - t0) / (it + 1)
     n_newton = gpu['newton_steps']
     cpu_round = t_samp + t_bc + max(1, n_newton) * t_it
@@ -423,7 +500,7 @@ def measure_config1(seed=0):
             "cpu_bicounts_s": t_bc, "cpu_newton_iter_s": t_it,
             "speedup_round": cpu_round / gpu['wall_s'],
             "note": "CPU round = sampler + numpy bicounts + GPU's Newton iteration count x "
-                    "measured numpy iteration time (2 iterations timed)"}
+                    "measured numpy iteration time (4 iterations timed)"}
 
 
 def copy_calibration(mib):
@@ -491,8 +568,15 @@ def main():
                     help="sweeps to sample the 2^20 target sequences")
     ap.add_argument('--rounds', type=int, default=2,
                     help="Mi3 inference rounds to time (0 = skip)")
-    ap.add_argument('--config1', action='store_true',
-                    help="also time the L=99 q=4 round on the GPU and its CPU reference path")
+    ap.add_argument('--round-equiltime', default='8',
+                    help="equiltime of the timed rounds (launches of 16 sweeps, or 'auto')")
+    ap.add_argument('--round-nsteps', type=int, default=0,
+                    help="nsteps of the timed rounds (default 16*L)")
+    ap.add_argument('--hiv-rounds', type=int, default=0,
+                    help="also run the reference's HIV example (pbs.sh settings) for this "
+                         "many rounds and compare with its published convergence")
+    ap.add_argument('--no-config1', action='store_true',
+                    help="skip the L=99 q=4 round on the GPU and its CPU reference path")
     ap.add_argument('--kernels', action='store_true',
                     help="also time bicounts and energies standalone on 2^20 sequences")
     ap.add_argument('--batch-rounds', type=int, default=0,
@@ -531,6 +615,15 @@ def main():
         g.wait()
         dt = time.perf_counter() - t0
         kms, kcnt = g.kernel_ms('mcmc')
+        # the same launches' coupling staging alone (same sites, chunks and
+        # launch shape; walker state untouched)
+        from mi3gpu._lib import lib, check
+        for _ in range(3):
+            check(g._ctx, lib().mi3_stage_probe(g._ctx, 1))
+            check(g._ctx, lib().mi3_stage_probe(g._ctx, 2))
+        g.wait()
+        p1_ms, _ = g.kernel_ms('stageProbe')
+        p2_ms, _ = g.kernel_ms('stageProbe2')
         # energies of the final ensemble (sanity)
         g.calcEnergies('main')
         E = g.getBuf('E main').read()
@@ -543,6 +636,27 @@ def main():
     value = su_per_step * a.steps / dt
     alg_bytes = su_per_step * (L + 1)
     achieved = alg_bytes / (kms * 1e-3) / 1e9
+    key = (L, q, N, a.sweeps_per_step, a.wgsize)
+    # bytes every work-group copies from L2 into LDS per launch: the padded
+    # coupling chunks of each step's site
+    chunk_bytes = _pf_class(q) * 512 * 16
+    nchunk = (L + 31) // 32
+    wgs = N // a.wgsize
+    stage_bytes = wgs * nsteps * nchunk * chunk_bytes
+    onchip = {
+        "binding": "on-chip: VALU issue (gathers) and the L2->LDS coupling staging; HBM "
+                   "traffic is 0.12 of the algorithmic bytes",
+        "staging_bytes_per_launch": stage_bytes,
+        "staging_TBps": stage_bytes / (kms * 1e-3) / 1e12,
+        "staging_alone_ms": p1_ms,
+        "staging_alone_TBps": stage_bytes / (p1_ms * 1e-3) / 1e12,
+        "staging_alone_2inflight_ms": p2_ms,
+        "staging_alone_2inflight_TBps": stage_bytes / (p2_ms * 1e-3) / 1e12,
+        "sweep_over_staging_alone": kms / p1_ms,
+        "lds_gather_bytes_per_launch": su_per_step * 8 * (L - 1),
+        "lds_gather_TBps": su_per_step * 8 * (L - 1) / (kms * 1e-3) / 1e12,
+        "valu_busy_profiled": MEASURED_VALU_BUSY.get(key),
+    }
     out = {
         "metric": METRIC,
         "value": value,
@@ -563,13 +677,13 @@ def main():
                    "walkers_per_workgroup": a.wgsize,
                    "mean_energy_after": float(np.mean(E))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(
-                         (L, q, N, a.sweeps_per_step, a.wgsize)),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get(key),
                      "kernel": "k_sweep", "kernel_ms": kms, "launches": kcnt,
                      "alg_bytes_per_launch": alg_bytes,
                      "achievable": hbm_copy, "frac_of_achievable": achieved / hbm_copy,
                      "achievable_kind": hbm_kind,
-                     "achievable_all": hbm_rates},
+                     "achievable_all": hbm_rates,
+                     "onchip": onchip},
     }
     if a.ess_sweeps > 0 or a.rounds > 0 or a.made:
         tseqs, target = sample_target(J, L, q, 2 ** 20, a.target_sweeps, 4, a.wgsize)
@@ -580,10 +694,13 @@ def main():
                                                    train_steps=a.made_train_steps)
         del tseqs
         if a.rounds > 0:
-            out["round"] = measure_rounds(target, L, q, N, a.rounds, 6)
+            out["round"] = measure_rounds(target, L, q, N, a.rounds, 6, a.round_equiltime,
+                                          a.round_nsteps or None)
     if a.kernels:
         out["kernels"] = measure_kernels()
-    if a.config1:
+    if a.hiv_rounds > 0:
+        out["hiv_reference_run"] = measure_hiv(a.hiv_rounds)
+    if not a.no_config1 and not a.no_cpu:
         out["config1_round"] = measure_config1()
     if a.batch_rounds > 0:
         out["batch"] = measure_batch(a.batch_rounds)

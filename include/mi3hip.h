@@ -163,6 +163,15 @@ int mi3_init_mcmc(mi3_ctx *ctx, uint32_t nsteps, uint64_t rng_offset, uint64_t r
  * (mcmcGPU.py:460-482) and the metropolis kernel (mcmc.cl:415-458). */
 int mi3_run_mcmc(mi3_ctx *ctx, float beta);
 
+/* Measurement probe with no counterpart in the reference: the coupling-row
+ * staging of one mi3_run_mcmc call alone (the same chunk copies into LDS, by
+ * the same launch shape, for the sites the next call would visit, with one
+ * barrier per chunk and no gathers, reduction or accept).  inflight = 1
+ * awaits each chunk's copy before the following barrier, as the sweep does;
+ * inflight = 2 keeps one copy in flight across it.  Recorded in the profile
+ * as "stageProbe" / "stageProbe2"; the walker state is not touched. */
+int mi3_stage_probe(mi3_ctx *ctx, int inflight);
+
 /* Number of MCMC steps executed so far on this context (the global step
  * counter of the random streams). */
 int mi3_mcmc_step_count(mi3_ctx *ctx, uint64_t *steps);

@@ -507,6 +507,45 @@ k_sweep_batch(const BatchModel* __restrict__ models, const BatchTile* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// Coupling-row staging alone: the sweep's per-step copies of J[pos] (the same
+// chunks of the same sites, from the same chunked layout, by the same waves
+// and instructions, one barrier per chunk) with no gathers, reduction or
+// accept.  With INFLIGHT = 1 each chunk's copy is awaited before the barrier
+// that follows its issue, as in the sweep; with INFLIGHT = 2 the copy of the
+// next chunk stays in flight across the barrier (the LDS contents are not
+// used).  Its time is the staging ceiling of the sweep's launch shape.
+template <int PF, int NWAVES, int INFLIGHT>
+__global__ __launch_bounds__(64 * NWAVES) void k_stage_probe(const float* __restrict__ Ju,
+                                                             const uint32_t* __restrict__ pos_list,
+                                                             uint32_t L, uint32_t nchunk,
+                                                             uint32_t nsteps) {
+    constexpr int NT = 64 * NWAVES;
+    constexpr int DMA_T = NT < 512 ? NT : 512;
+    constexpr uint32_t chb = PF * DMA_T * 16;
+    extern __shared__ f4v lds4[];
+    char* ldsb = reinterpret_cast<char*>(lds4);
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const __amdgpu_buffer_rsrc_t rJ = make_rsrc(Ju, L * nchunk * chb);
+    const uint32_t voff = tid * 16;
+    uint32_t buf = 0;
+    for (uint32_t t = 0; t < nsteps; t++) {
+        const uint32_t pos = __builtin_amdgcn_readfirstlane(pos_list[t]);
+        for (uint32_t k = 0; k < nchunk; k++) {
+            if (wave < DMA_T / 64)
+                chunk_dma<PF, DMA_T>(rJ, voff, (pos * nchunk + k) * chb, ldsb + buf * chb, wave);
+            if constexpr (INFLIGHT == 1)
+                wait_vmem();
+            else if (wave < DMA_T / 64)
+                __builtin_amdgcn_s_waitcnt(0x0F70 | (PF & 0xF) | ((PF >> 4) << 14));
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+    wait_vmem();
+}
+
+// ---------------------------------------------------------------------------
 // Batch energies E(s) = sum_{i<m} J_{i,m}(s_i, s_m) of nseq sequences.
 // Restates getEnergies (mcmc.cl:249-345); same layout and staging as the
 // sweep, streaming for each row i only the chunks holding m > i.

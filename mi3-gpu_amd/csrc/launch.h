@@ -76,6 +76,10 @@ struct BatchTile {
 hipError_t launch_sweep(const SweepArgs& a, hipStream_t st);
 hipError_t launch_energies(const EnergyArgs& a, hipStream_t st);
 
+// The sweep's coupling-row staging alone (k_stage_probe) for the sites of
+// a.pos, one work-group per 512 walkers; inflight 1 or 2 chunk copies.
+hipError_t launch_stage_probe(const SweepArgs& a, int inflight, hipStream_t st);
+
 // State words per lane (4 residues each) for L sites: ceil(L / 128).
 inline uint32_t state_words(uint32_t L) { return (L + 127) / 128; }
 

@@ -154,6 +154,31 @@ hipError_t launch_sweep_batch(uint32_t kw, uint32_t pf, uint32_t wpb, const Batc
     return hipErrorInvalidValue;
 }
 
+namespace {
+template <int PF, int INF>
+hipError_t stage_probe_pf(const SweepArgs& a, hipStream_t st) {
+    constexpr int NW = 16;
+    auto kern = k_stage_probe<PF, NW, INF>;
+    size_t lds = 2ull * chunk_floats(a.q) * sizeof(float);
+    hipError_t e = set_lds(kern, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(a.nwalk / 512), dim3(64 * NW), lds, st, a.Ju, a.pos, a.L,
+                       a.nchunk, a.nsteps);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_stage_probe(const SweepArgs& a, int inflight, hipStream_t st) {
+    if (a.nwalk % 512 || a.q > MAX_Q || a.L > MAX_L || a.L < 2) return hipErrorInvalidValue;
+    const uint32_t pf = pf_class(a.q);
+#define MI3_PCASE(PF_)                                                             \
+    if (pf == PF_) return inflight == 2 ? stage_probe_pf<PF_, 2>(a, st)             \
+                                        : stage_probe_pf<PF_, 1>(a, st);
+    MI3_PCASE(1) MI3_PCASE(4) MI3_PCASE(7) MI3_PCASE(10)
+#undef MI3_PCASE
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_energies(const EnergyArgs& a, hipStream_t st) {
     if (a.q > MAX_Q || a.L > MAX_L || a.L < 2) return hipErrorInvalidValue;
     if (a.nseq == 0) return hipSuccess;

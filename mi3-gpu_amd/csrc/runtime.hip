@@ -650,6 +650,33 @@ int mi3_run_mcmc(mi3_ctx* c, float beta) {
     return MI3_OK;
 }
 
+int mi3_stage_probe(mi3_ctx* c, int inflight) {
+    if (!c) return MI3_EINVAL;
+    if (!c->mcmc) return fail(c, MI3_ESTATE, "MCMC not initialized");
+    if (inflight != 1 && inflight != 2) return fail(c, MI3_EINVAL, "inflight must be 1 or 2");
+    float* Ju;
+    int rc = chunked_of(c, "J", &Ju);
+    if (rc) return rc;
+    {
+        Scope s(c, "positions", (size_t)c->nsteps * 4);
+        hipLaunchKernelGGL(k_positions, dim3(grid1(c->nsteps)), dim3(256), 0, c->stream, c->pos,
+                           c->nsteps, c->L, c->k0, c->k1, c->step);
+        HIPCHK(c, hipGetLastError());
+    }
+    SweepArgs a{};
+    a.Ju = Ju;
+    a.pos = c->pos;
+    a.nwalk = c->N;
+    a.L = c->L;
+    a.q = c->q;
+    a.nchunk = c->nchunk;
+    a.nsteps = c->nsteps;
+    const size_t bytes = (size_t)(c->N / 512) * c->nsteps * c->nchunk * chunk_floats(c->q) * 4;
+    Scope s(c, inflight == 1 ? "stageProbe" : "stageProbe2", bytes);
+    HIPCHK(c, launch_stage_probe(a, inflight, c->stream));
+    return MI3_OK;
+}
+
 int mi3_mcmc_step_count(mi3_ctx* c, uint64_t* steps) {
     if (!c || !steps) return MI3_EINVAL;
     *steps = c->step;

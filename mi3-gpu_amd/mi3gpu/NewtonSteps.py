@@ -399,10 +399,16 @@ def MCMCstep(runName, Jstep, couplings, param, gpus, log):
     param.last_round_ssr = ssr
     # per-round timing record (wall s; not part of the reference's outputs)
     if 'round_stats' in param:
-        param.round_stats.append({'run': runName, 'wall_s': time.time() - tstart,
+        wall = time.time() - tstart
+        ferr, _, _ = bimarg_stats(bimarg_target, bimarg_model)
+        X = np.sum(couplings * (bimarg_model - indepF(bimarg_model)))
+        Xo = np.sum(couplings * (bimarg_target - indepF(bimarg_target)))
+        param.round_stats.append({'run': runName, 'wall_s': wall,
                                   'mcmc_s': dt, 'status_write_s': t_write,
                                   'newton_s': param.last_newton_time, 'newton_steps': Jsteps,
-                                  'equil_launches': equilsteps, 'ssr': float(ssr)})
+                                  'jstep': int(Jstep), 'equil_launches': int(equilsteps),
+                                  'ssr': float(ssr), 'ferr': float(ferr), 'X': float(X),
+                                  'Xo': float(Xo)})
     return Jstep + Jsteps, seqs, energies, newJ
 
 
