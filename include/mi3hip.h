@@ -209,6 +209,24 @@ int mi3_bicounts_to_bimarg(mi3_ctx *ctx, uint32_t nseq);
  * once, so the result does not depend on summation order. */
 int mi3_weighted_marg(mi3_ctx *ctx, const char *seqbuf, const char *wbuf, uint32_t nseq);
 
+/* Allocation-free variant of mi3_weighted_marg.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The caller passes device pointers: `seq` site-major with row
+ * stride `ld` bytes (a multiple of 16, nseq <= ld), `w` nseq floats, and the
+ * output `bi` of (L(L-1)/2) * q * q floats; the result equals
+ * mi3_weighted_marg's bit for bit.  Scratch is a library-owned static device
+ * array of MI3_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends:
+ * the call allocates nothing and rejects nseq > MI3_NOALLOC_MAX_SEQ with
+ * MI3_EINVAL.  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (from several contexts or streams) race; calls on one
+ * context's stream are ordered and safe. */
+#define MI3_NOALLOC_MAX_SEQ (1u << 20)
+#define MI3_NOALLOC_SCRATCH_BYTES \
+    ((size_t)MI3_NOALLOC_MAX_SEQ * 8 + 256 + 512 * 16 + 512 + ((size_t)8 << 21))
+int mi3_weighted_marg_noalloc(mi3_ctx *ctx, const uint8_t *seq, uint32_t ld, const float *w,
+                              uint32_t nseq, float *bi);
+
 /* Divide each pair block of "bi" by its sum (renormalize_bimarg,
  * mcmcGPU.py:663-668; merge_bimarg on one device, node_manager.py:283-287). */
 int mi3_renormalize_bimarg(mi3_ctx *ctx);

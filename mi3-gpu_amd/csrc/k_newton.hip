@@ -240,7 +240,9 @@ __global__ void k_weights_fixed(const float* __restrict__ w, uint32_t n,
 // so the row-i residues are wave-uniform (scalar) loads.  The 16 waves take
 // groups of 16 sequences in turn, two groups per iteration with all their
 // loads issued before the first add.  The part's histograms are added into
-// the uint64 accumulator (zeroed beforehand) with integer atomics.
+// the uint64 accumulator (zeroed beforehand) with integer atomics; the
+// accumulator holds the pairs from pair_lo on (units are ordered by row,
+// then partner group, so a run of units covers a contiguous pair range).
 constexpr uint32_t WM_THREADS = 1024;
 constexpr uint32_t WM_COLS = 32;
 
@@ -286,7 +288,8 @@ __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __r
                                                               uint32_t L, uint32_t q,
                                                               const uint32_t* __restrict__ units,
                                                               uint32_t nsplit,
-                                                              unsigned long long* __restrict__ acc) {
+                                                              unsigned long long* __restrict__ acc,
+                                                              uint32_t pair_lo) {
     extern __shared__ unsigned long long hist64[];
     constexpr uint32_t NW = WM_THREADS / 64;
     const uint32_t unit = units[blockIdx.x / nsplit], part = blockIdx.x % nsplit;
@@ -335,7 +338,8 @@ __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __r
         const uint32_t c = e & (WM_COLS - 1), ab = e / WM_COLS;
         const uint32_t mm = cp * WM_COLS + c;
         const unsigned long long v = hist64[e];
-        if (mm > i && mm < L && v) atomicAdd(&acc[(size_t)pair_index(i, mm, L) * qq + ab], v);
+        if (mm > i && mm < L && v)
+            atomicAdd(&acc[(size_t)(pair_index(i, mm, L) - pair_lo) * qq + ab], v);
     }
 }
 

@@ -2,10 +2,12 @@
 // C ABI declared in include/mi3hip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +18,10 @@
 #include "k_bicount.hip"
 
 using namespace mi3;
+
+// Library-owned scratch of the allocation-free entry point (one copy per
+// device; see mi3_weighted_marg_noalloc).
+static __device__ __align__(256) unsigned char scratch[MI3_NOALLOC_SCRATCH_BYTES];
 
 namespace mi3 {
 hipError_t launch_unpack_chunked(const float* J, float* out, uint32_t L, uint32_t q,
@@ -93,6 +99,11 @@ struct mi3_ctx {
     uint32_t* wm_units = nullptr;
     uint32_t n_units = 0;
     uint32_t wm_split = 1;
+    std::vector<uint32_t> unit_pair_lo, unit_pair_hi;  // pair range [lo, hi) of each unit
+    // this device's copy of the allocation-free scratch array (null, with
+    // the reason in scratch_err, if its address could not be resolved)
+    char* scratch = nullptr;
+    std::string scratch_err;
     uint8_t* seq1 = nullptr;                  // one sequence (fill_seqs)
     // host staging
     std::vector<Pinned> pins;
@@ -106,6 +117,25 @@ struct mi3_ctx {
 namespace {
 
 constexpr uint32_t RED_BLOCKS = 512;
+
+// Scratch of mi3_weighted_marg_noalloc: one static device array per GPU the
+// library runs on, held until the process ends.  Layout: fixed-point weights
+// (8 B per sequence), reduction partials, the reduced (sum, sum of squares),
+// the bad-total flag, and a uint64 accumulator for a run of pair blocks.
+constexpr size_t NA_WQ_BYTES = (size_t)MI3_NOALLOC_MAX_SEQ * 8 + 256;
+constexpr size_t NA_RED_OFF = NA_WQ_BYTES;
+constexpr size_t NA_REDOUT_OFF = NA_RED_OFF + RED_BLOCKS * 2 * sizeof(double);
+constexpr size_t NA_BAD_OFF = NA_REDOUT_OFF + 256;
+constexpr size_t NA_ACC_OFF = NA_BAD_OFF + 256;
+constexpr uint64_t NA_ACC_BINS = 1ull << 21;
+constexpr size_t NA_BYTES = NA_ACC_OFF + NA_ACC_BINS * 8;
+static_assert(NA_BYTES == MI3_NOALLOC_SCRATCH_BYTES, "scratch size declared in mi3hip.h");
+
+// Device address of `scratch` on each device, resolved once per device with
+// that device current (the array's name in host code is only a stand-in).
+constexpr int NA_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[NA_MAX_DEV] = {};
 
 int fail(mi3_ctx* c, int code, const std::string& msg) {
     if (c)
@@ -292,6 +322,23 @@ int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t s
     c->k0 = (uint32_t)seed;
     c->k1 = (uint32_t)(seed >> 32);
     c->profile = profile != 0;
+    {
+        // this device's address of the scratch array, resolved once per
+        // device with the device current
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (device >= NA_MAX_DEV) {
+            c->scratch_err = "allocation-free scratch: device index beyond the table";
+        } else if (!g_scratch_ptr[device]) {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e == hipSuccess && p)
+                g_scratch_ptr[device] = static_cast<char*>(p);
+            else
+                c->scratch_err = std::string("hipGetSymbolAddress(scratch): ") +
+                                 hipGetErrorString(e);
+        }
+        if (device < NA_MAX_DEV) c->scratch = g_scratch_ptr[device];
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(nullptr, MI3_EHIP, "stream creation failed");
@@ -306,7 +353,15 @@ int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t s
     std::vector<uint32_t> units;
     const uint32_t ngrp = (L + WM_COLS - 1) / WM_COLS;
     for (uint32_t i = 0; i + 1 < L; i++)
-        for (uint32_t cp = (i + 1) / WM_COLS; cp < ngrp; cp++) units.push_back((i << 8) | cp);
+        for (uint32_t cp = (i + 1) / WM_COLS; cp < ngrp; cp++) {
+            units.push_back((i << 8) | cp);
+            // pairs (i, m), m in [max(i+1, 32 cp), min(L-1, 32 cp + 31)]
+            const uint32_t m0 = std::max(i + 1, cp * WM_COLS);
+            const uint32_t m1 = std::min(L - 1, cp * WM_COLS + WM_COLS - 1);
+            const uint32_t row0 = i * (2 * L - i - 1) / 2;
+            c->unit_pair_lo.push_back(row0 + (m0 - i - 1));
+            c->unit_pair_hi.push_back(row0 + (m1 - i - 1) + 1);
+        }
     c->n_units = (uint32_t)units.size();
     {
         double best = 0.0;
@@ -823,12 +878,68 @@ int mi3_weighted_marg(mi3_ctx* c, const char* seqbuf, const char* wbuf, uint32_t
     hipLaunchKernelGGL(k_weighted_marg, dim3(c->n_units * c->wm_split), dim3(WM_THREADS), lds,
                        c->stream, (const uint8_t*)sb->d, (uint32_t)ld, nseq,
                        (const uint64_t*)c->wq, c->L, c->q, (const uint32_t*)c->wm_units,
-                       c->wm_split, c->wacc);
+                       c->wm_split, c->wacc, 0u);
     hipLaunchKernelGGL(k_fixed_to_float, dim3(grid1(nbins)), dim3(256), 0, c->stream,
                        (const unsigned long long*)c->wacc, (float*)bb->d, nbins,
                        (const float*)c->red_out, (const uint32_t*)c->wbad);
     HIPCHK(c, hipGetLastError());
     bb->version++;
+    return MI3_OK;
+}
+
+int mi3_weighted_marg_noalloc(mi3_ctx* c, const uint8_t* seq, uint32_t ld, const float* w,
+                              uint32_t nseq, float* bi) {
+    if (!c) return MI3_EINVAL;
+    if (!seq || !w || !bi) return fail(c, MI3_EINVAL, "null device pointer");
+    if (nseq > ld) return fail(c, MI3_EINVAL, "nseq exceeds the sequence row stride");
+    if (ld % 16) return fail(c, MI3_EINVAL, "sequence buffer row stride must be a multiple of 16");
+    if (nseq > MI3_NOALLOC_MAX_SEQ)
+        return fail(c, MI3_EINVAL,
+                    "nseq = " + std::to_string(nseq) + " exceeds the allocation-free limit of " +
+                        std::to_string(MI3_NOALLOC_MAX_SEQ) + " sequences");
+    if (!c->scratch) return fail(c, MI3_EHIP, c->scratch_err);
+    // scratch regions (this device's copy, by its resolved address)
+    uint64_t* wq = reinterpret_cast<uint64_t*>(c->scratch);
+    double* red = reinterpret_cast<double*>(c->scratch + NA_RED_OFF);
+    float* red_out = reinterpret_cast<float*>(c->scratch + NA_REDOUT_OFF);
+    uint32_t* bad = reinterpret_cast<uint32_t*>(c->scratch + NA_BAD_OFF);
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(c->scratch + NA_ACC_OFF);
+    const uint64_t qq = (uint64_t)c->q * c->q;
+    const size_t lds = qq * WM_COLS * sizeof(unsigned long long);
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_weighted_marg),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // sum of the weights, then fixed-point weights (every kernel below
+    // writes what it later reads: nothing in the scratch needs clearing
+    // except the accumulator, cleared per run of units)
+    uint32_t nb = grid1(nseq);
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(256), 0, c->stream, w, nseq, 1, red);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, c->stream, (const double*)red, nb, 1,
+                       red_out);
+    hipLaunchKernelGGL(k_weights_fixed, dim3(grid1(nseq ? nseq : 1)), dim3(256), 0, c->stream, w,
+                       nseq, (const float*)red_out, wq, bad);
+    HIPCHK(c, hipGetLastError());
+    // runs of units whose pair blocks fit the accumulator: clear it (a fill
+    // kernel, not a memset, so the sequence can be graph-captured), count,
+    // and convert the run's bins into their place in `bi`
+    uint32_t u0 = 0;
+    while (u0 < c->n_units) {
+        const uint32_t p_lo = c->unit_pair_lo[u0];
+        uint32_t u1 = u0 + 1;
+        while (u1 < c->n_units && (uint64_t)(c->unit_pair_hi[u1] - p_lo) * qq <= NA_ACC_BINS) u1++;
+        const uint64_t nbins = (uint64_t)(c->unit_pair_hi[u1 - 1] - p_lo) * qq;
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid1(2 * nbins)), dim3(256), 0, c->stream,
+                           reinterpret_cast<uint32_t*>(acc), 0u, 2 * nbins);
+        hipLaunchKernelGGL(k_weighted_marg, dim3((u1 - u0) * c->wm_split), dim3(WM_THREADS), lds,
+                           c->stream, seq, ld, nseq, (const uint64_t*)wq, c->L, c->q,
+                           (const uint32_t*)(c->wm_units + u0), c->wm_split, acc, p_lo);
+        hipLaunchKernelGGL(k_fixed_to_float, dim3(grid1(nbins)), dim3(256), 0, c->stream,
+                           (const unsigned long long*)acc, bi + (size_t)p_lo * qq, nbins,
+                           (const float*)red_out, (const uint32_t*)bad);
+        HIPCHK(c, hipGetLastError());
+        u0 = u1;
+    }
     return MI3_OK;
 }
 

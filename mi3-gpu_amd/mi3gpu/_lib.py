@@ -53,6 +53,7 @@ SIGNATURES = {
     'mi3_bicount_split': (_i, [_P, _u32, _c.POINTER(_u32)]),
     'mi3_bicounts_to_bimarg': (_i, [_P, _u32]),
     'mi3_weighted_marg': (_i, [_P, _S, _S, _u32]),
+    'mi3_weighted_marg_noalloc': (_i, [_P, _P, _u32, _P, _u32, _P]),
     'mi3_renormalize_bimarg': (_i, [_P]),
     'mi3_min_buf': (_i, [_P, _S, _u32]),
     'mi3_weight_stats': (_i, [_P, _S, _u32]),
