@@ -222,97 +222,263 @@ __global__ void k_relu_grad(uint64_t n, const float* __restrict__ dh,
     if (outb) outb[i] = (bf16)v;
 }
 
-// Autoregressive sampling, site j: logits_j = W3[q*j .. q*j+q) h2 + b3,
-// s_j ~ softmax(logits_j) by inversion of u = Philox((j, walker, counter,
-// TAG_MADE)) -- one thread per walker b;
-// nll[b] -= log p(s_j) (written at j = 0, accumulated after); s_j written to
-// row j of the site-major output.
-// The site's q rows of W3 (bf16) are staged in LDS.
-__global__ __launch_bounds__(256) void k_made_site(uint32_t j, const bf16* __restrict__ h2b,
-                                                   uint32_t H, const bf16* __restrict__ W3b,
-                                                   const float* __restrict__ b3, uint32_t q,
-                                                   uint32_t B, uint32_t k0, uint32_t k1,
-                                                   uint32_t counter, uint32_t walker_base,
-                                                   uint8_t* __restrict__ sout, uint32_t ld,
-                                                   float* __restrict__ nll) {
-    extern __shared__ float w3s[];   // q x H floats
-    for (uint32_t e = threadIdx.x; e < q * H; e += blockDim.x)
-        w3s[e] = (float)W3b[(size_t)q * j * H + e];
-    __syncthreads();
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    float z[32];
+// Fused autoregressive pass over all L sites for tiles of MF_T = 32 walkers,
+// one work-group of 8 waves per tile: with draw != 0 it samples s_j by
+// inversion of u = Philox((j, walker_base + b, counter, TAG_MADE)) and
+// writes it to row j of seq; with draw == 0 it reads s_j from seq (teacher
+// forcing).  Either way nll[b] = -sum_j log p(s_j).
+//
+// The degree masks make each hidden unit final once: unit k (degree
+// m_k = 1 + k mod (L-1)) of layer 1 sees sites i < m_k, unit k of layer 2
+// sees layer-1 units of degree <= m_k, and site j's logits see layer-2 units
+// of degree <= j.  So at site j the layer-1 units of degree j have all their
+// inputs, the layer-2 units of degree j can be formed from them, and the
+// logits need nothing newer: each unit is computed once per walker, at its
+// degree, instead of the whole hidden layer at every site (L times less
+// matrix-core work than a GEMM per site).
+//
+// Per site j: (A) layer-1 units of degree j, kept as float32 sums in
+// registers (thread t owns unit t for all 32 walkers), go to LDS as
+// bf16(relu); (B) layer-2 units of degree j: [32 walkers x H] x W2b rows of
+// those units on v_mfma_f32_32x32x16_bf16, the k-steps dealt to 4 waves
+// (wave 0 starts from the bias), the partials summed by all threads in a
+// fixed order, then bf16(relu(.)) into LDS; (C) site j's q logits the same
+// way from the layer-2 units, W3b rows q*j..q*j+q-1 and b3; (D) one thread
+// per walker: softmax, draw or read, nll, while wave 1 draws the next site's
+// uniforms; (E) the float32 layer-1 sums of units of degree > j add W1t's
+// row of the site's letter (the sums are formed in site order, as the
+// teacher-forced embedding forms them).
+//
+// Everything a site reads from memory except the drawn letters is known a
+// site ahead and loaded into registers while the previous site computes:
+// the B operands and biases of (B) and (C) right after the products that
+// use them, this thread's W1t entries for every letter of the next site
+// (staged in LDS at the next site's start for (E)) and, when teacher
+// forcing, the next site's letters.  Requires H % 16 == 0, H <= 512,
+// q <= 25.
+constexpr int MF_T = 32, MF_WAVES = 8, MF_THREADS = 64 * MF_WAVES, MF_HMAX = 512;
+constexpr int MF_QMAX = 25;
+constexpr int MF_MW = 4;                        // waves that run the products
+constexpr int MF_HP = MF_HMAX + 8;              // padded LDS row (bf16) of a walker's units
+constexpr int MF_KS = MF_HMAX / 16 / MF_MW;     // k-steps of 16 per product wave
+static_assert(MF_THREADS == MF_HMAX, "one layer-1 unit per thread");
+
+struct MadeFusedLds {
+    bf16 h1[MF_T][MF_HP];                  // bf16(relu(layer 1)), final units only
+    bf16 h2[MF_T][MF_HP];                  // bf16(relu(layer 2)), final units only
+    float part[MF_MW][MF_T][MF_T + 1];     // per-wave partial products
+    float z[MF_T][MF_T + 1];               // the site's logits
+    float w1[MF_QMAX][MF_HMAX];            // W1t entries of the site by letter
+    float u[2][MF_T];                      // uniforms of this / the next site
+    uint32_t s[MF_T];                      // the site's letters
+};
+
+// Operands of one product wave's share of a [32 x H] x [H x 32] product: the
+// B operand (this lane's 8 k-values of column lane%32 for each of the wave's
+// k-steps; zero for a negative row or k past H) and, for wave 0, the bias of
+// the column (the accumulator's start).
+struct MadeB {
+    bf16x8 b[MF_KS];
+    float bias;
+};
+
+__device__ __forceinline__ void made_load_b(MadeB& r, const bf16* __restrict__ W,
+                                            const float* __restrict__ bias, uint32_t H, int row,
+                                            int lg, int wave) {
+    const bf16x8 zero8 = {};
 #pragma unroll
-    for (uint32_t a = 0; a < 32; a++) z[a] = 0.f;
-    const bf16* h = h2b + (size_t)b * H;
-    for (uint32_t k = 0; k < H; k += 8) {
-        const bf16x8 hv = *reinterpret_cast<const bf16x8*>(h + k);
-        float hf[8];
+    for (int t = 0; t < MF_KS; t++) {
+        const uint32_t k0 = 16 * (wave + MF_MW * t);
+        r.b[t] = (row >= 0 && k0 < H)
+            ? *reinterpret_cast<const bf16x8*>(W + (size_t)row * H + k0 + 8 * lg) : zero8;
+    }
+    r.bias = (wave == 0 && row >= 0) ? bias[row] : 0.f;
+}
+
+// part[wave] = bias (wave 0) + X[32 x H] . (this wave's k-steps of b)
+__device__ __forceinline__ void made_product(const bf16 (*X)[MF_HP], const MadeB& b, uint32_t H,
+                                             float (*part)[MF_T + 1], int lr, int lg, int wave) {
+    f32x16 acc;
 #pragma unroll
-        for (int e = 0; e < 8; e++) hf[e] = (float)hv[e];
+    for (int e = 0; e < 16; e++) acc[e] = b.bias;
 #pragma unroll
-        for (uint32_t a = 0; a < 32; a++) {
-            if (a < q) {
-                float acc = 0.f;
+    for (int t = 0; t < MF_KS; t++) {
+        const uint32_t k0 = 16 * (wave + MF_MW * t);
+        if (k0 < H) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&X[lr][k0 + 8 * lg]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b.b[t], acc, 0, 0, 0);
+        }
+    }
+    // D layout: value e of lane l is row 8*(e/4) + 4*(l/32) + e%4, column l%32
 #pragma unroll
-                for (int e = 0; e < 8; e++) acc += hf[e] * w3s[a * H + k + e];
-                z[a] += acc;
+    for (int e = 0; e < 16; e++) part[8 * (e >> 2) + 4 * lg + (e & 3)][lr] = acc[e];
+}
+
+__device__ __forceinline__ float made_part_sum(const float (*part)[MF_T][MF_T + 1], uint32_t r,
+                                               uint32_t c) {
+    return ((part[0][r][c] + part[1][r][c]) + part[2][r][c]) + part[3][r][c];
+}
+
+__global__ __launch_bounds__(MF_THREADS) void k_made_fused(
+    uint32_t L, uint32_t q, uint32_t H, uint32_t B, const float* __restrict__ W1t,
+    const float* __restrict__ b1, const bf16* __restrict__ W2b, const float* __restrict__ b2,
+    const bf16* __restrict__ W3b, const float* __restrict__ b3, uint32_t k0, uint32_t k1,
+    uint32_t counter, uint32_t walker_base, uint8_t* __restrict__ seq, uint32_t ld,
+    float* __restrict__ nll, int draw) {
+    extern __shared__ __attribute__((aligned(16))) char made_lds_raw[];
+    MadeFusedLds& sm = *reinterpret_cast<MadeFusedLds*>(made_lds_raw);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int lr = lane & 31, lg = lane >> 5;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool mwave = wave < MF_MW;
+    const uint32_t tile0 = blockIdx.x * MF_T;
+    const uint32_t Lm1 = L - 1;
+    // units of degree jj: k = (jj - 1) + c (L - 1), c = 0, 1, ...
+    auto n_units = [&](uint32_t jj) -> uint32_t {
+        return (jj >= 1 && jj < L && jj - 1 < H) ? (H - (jj - 1) + Lm1 - 1) / Lm1 : 0u;
+    };
+    const uint32_t k = tid;                       // this thread's layer-1 unit
+    const uint32_t deg = k < H ? 1 + k % Lm1 : 0xffffffffu;
+    float h1r[MF_T];
+    {
+        const float v = k < H ? b1[k] : 0.f;
+#pragma unroll
+        for (int w = 0; w < MF_T; w++) h1r[w] = v;
+    }
+    for (uint32_t e = tid; e < MF_T * MF_HP; e += MF_THREADS) {
+        (&sm.h1[0][0])[e] = (bf16)0.f;
+        (&sm.h2[0][0])[e] = (bf16)0.f;
+    }
+    // this thread's W1t entries for every letter of site jj (zero where the
+    // unit does not see the site)
+    float w1r[MF_QMAX];
+    auto load_w1 = [&](uint32_t jj) {
+        const bool sees = deg != 0xffffffffu && deg > jj && jj < L;
+#pragma unroll
+        for (uint32_t a = 0; a < MF_QMAX; a++)
+            w1r[a] = (sees && a < q) ? W1t[(size_t)(q * jj + a) * H + k] : 0.f;
+    };
+    MadeB w2r, w3r;
+    auto load_w2 = [&](uint32_t jj) {
+        const uint32_t nU = n_units(jj);
+        const int row = (mwave && nU <= MF_T && (uint32_t)lr < nU) ? (int)((jj - 1) + lr * Lm1)
+                                                                   : -1;
+        made_load_b(w2r, W2b, b2, H, row, lg, wave);
+    };
+    auto load_w3 = [&](uint32_t jj) {
+        const int row = (mwave && jj < L && (uint32_t)lr < q) ? (int)(q * jj + lr) : -1;
+        made_load_b(w3r, W3b, b3, H, row, lg, wave);
+    };
+    // uniforms of site jj for the tile's walkers (wave 1, lanes < 32)
+    auto draw_u = [&](uint32_t jj) {
+        if (draw && wave == 1 && lane < MF_T && jj < L) {
+            const uint4 r = philox4x32_10(make_uint4(jj, walker_base + tile0 + lane, counter,
+                                                     TAG_MADE), k0, k1);
+            sm.u[jj & 1][lane] = (float)(r.x >> 8) * 0x1.0p-24f;
+        }
+    };
+    // teacher forcing: thread w < 32 holds the letter of walker w at the next site
+    uint32_t s_next = 0;
+    auto load_s = [&](uint32_t jj) {
+        const uint32_t b = tile0 + tid;
+        s_next = (!draw && tid < MF_T && b < B && jj < L) ? seq[(size_t)jj * ld + b] : 0u;
+    };
+    load_w2(0);
+    load_w3(0);
+    load_w1(0);
+    load_s(0);
+    draw_u(0);
+    float nacc = 0.f;    // thread w < 32: nll of walker tile0 + w
+
+    for (uint32_t j = 0; j < L; j++) {
+        // (A) layer-1 units of degree j are final; the site's W1t entries
+#pragma unroll
+        for (uint32_t a = 0; a < MF_QMAX; a++) sm.w1[a][k] = w1r[a];
+        if (deg == j) {
+#pragma unroll
+            for (int w = 0; w < MF_T; w++) sm.h1[w][k] = (bf16)fmaxf(h1r[w], 0.f);
+        }
+        __syncthreads();
+        // (B) layer-2 units of degree j
+        const uint32_t nU = n_units(j);
+        for (uint32_t c0 = 0; c0 < nU; c0 += MF_T) {
+            if (mwave) {
+                if (nU > MF_T) {   // more than one column tile: operands loaded here
+                    const uint32_t c = c0 + lr;
+                    made_load_b(w2r, W2b, b2, H, c < nU ? (int)((j - 1) + c * Lm1) : -1, lg,
+                                wave);
+                }
+                made_product(sm.h1, w2r, H, sm.part[wave], lr, lg, wave);
             }
+            __syncthreads();
+            for (uint32_t e = tid; e < MF_T * MF_T; e += MF_THREADS) {
+                const uint32_t r = e / MF_T, c = e % MF_T;
+                if (c0 + c < nU)
+                    sm.h2[r][(j - 1) + (c0 + c) * Lm1] =
+                        (bf16)fmaxf(made_part_sum(sm.part, r, c), 0.f);
+            }
+            __syncthreads();
+        }
+        load_w2(j + 1);
+        // (C) logits of site j (columns a < q), b3 included
+        if (mwave) made_product(sm.h2, w3r, H, sm.part[wave], lr, lg, wave);
+        load_w3(j + 1);
+        __syncthreads();
+        for (uint32_t e = tid; e < MF_T * MF_T; e += MF_THREADS) {
+            const uint32_t r = e / MF_T, c = e % MF_T;
+            if (c < q) sm.z[r][c] = made_part_sum(sm.part, r, c);
+        }
+        __syncthreads();
+        // (D) softmax and the letter, one thread per walker; wave 1 draws
+        // the next site's uniforms meanwhile
+        if (tid < MF_T) {
+            const uint32_t b = tile0 + tid;
+            float ez[MF_QMAX];
+            float mx = -INFINITY;
+#pragma unroll
+            for (uint32_t a = 0; a < MF_QMAX; a++)
+                if (a < q) mx = fmaxf(mx, sm.z[tid][a]);
+            float se = 0.f;
+#pragma unroll
+            for (uint32_t a = 0; a < MF_QMAX; a++) {
+                ez[a] = (a < q) ? expf(sm.z[tid][a] - mx) : 0.f;
+                se += ez[a];
+            }
+            uint32_t s = 0;
+            if (draw) {
+                const float uu = sm.u[j & 1][tid] * se;
+                float cm = 0.f;
+                s = q - 1;
+                bool found = false;
+#pragma unroll
+                for (uint32_t a = 0; a < MF_QMAX; a++) {
+                    if (a + 1 < q && !found) {
+                        cm += ez[a];
+                        if (uu < cm) {
+                            s = a;
+                            found = true;
+                        }
+                    }
+                }
+                if (b < B) seq[(size_t)j * ld + b] = (uint8_t)s;
+            } else {
+                s = s_next < q ? s_next : q - 1;
+            }
+            nacc += mx + logf(se) - sm.z[tid][s];
+            sm.s[tid] = s;
+        }
+        draw_u(j + 1);
+        load_s(j + 1);
+        __syncthreads();
+        // (E) sums of the layer-1 units that see site j (units that do not
+        // see it read a zero entry)
+        if (j + 1 < L) {
+#pragma unroll
+            for (int w = 0; w < MF_T; w++) h1r[w] += sm.w1[sm.s[w]][k];
+            load_w1(j + 1);
         }
     }
-    float mx = -INFINITY;
-#pragma unroll
-    for (uint32_t a = 0; a < 32; a++)
-        if (a < q) {
-            z[a] += b3[q * j + a];
-            mx = fmaxf(mx, z[a]);
-        }
-    float se = 0.f;
-#pragma unroll
-    for (uint32_t a = 0; a < 32; a++)
-        if (a < q) se += expf(z[a] - mx);
-    const uint4 r = philox4x32_10(make_uint4(j, walker_base + b, counter, TAG_MADE), k0, k1);
-    const float u = (float)(r.x >> 8) * 0x1.0p-24f * se;
-    // first a whose cumulative mass exceeds u (the last residue otherwise)
-    float c = 0.f;
-    uint32_t s = q - 1;
-    for (uint32_t a = 0; a + 1 < q; a++) {
-        c += expf(z[a] - mx);
-        if (u < c) {
-            s = a;
-            break;
-        }
-    }
-    float zs = z[0];
-#pragma unroll
-    for (uint32_t a = 0; a < 32; a++)
-        if (a == s) zs = z[a];
-    const float t = mx + logf(se) - zs;
-    nll[b] = j == 0 ? t : nll[b] + t;
-    sout[(size_t)j * ld + b] = (uint8_t)s;
-}
-
-// After sampling site j: h1[b][:] += W1t[(q*j + s_j)*H + :], h1b = bf16(relu(h1)).
-__global__ void k_made_h1_update(uint32_t j, const uint8_t* __restrict__ sout, uint32_t ld,
-                                 uint32_t B, uint32_t q, uint32_t H,
-                                 const float* __restrict__ W1t, float* __restrict__ h1,
-                                 bf16* __restrict__ h1b) {
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint64_t)B * H) return;
-    const uint32_t b = (uint32_t)(idx / H), k = (uint32_t)(idx - (uint64_t)b * H);
-    const float v = h1[idx] + W1t[(size_t)(q * j + sout[(size_t)j * ld + b]) * H + k];
-    h1[idx] = v;
-    h1b[idx] = (bf16)fmaxf(v, 0.f);
-}
-
-// h1 = b1 (start of sampling), h1b = bf16(relu(b1)).
-__global__ void k_made_h1_init(uint32_t B, uint32_t H, const float* __restrict__ b1,
-                               float* __restrict__ h1, bf16* __restrict__ h1b) {
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint64_t)B * H) return;
-    const float v = b1[idx % H];
-    h1[idx] = v;
-    h1b[idx] = (bf16)fmaxf(v, 0.f);
+    if (tid < MF_T && tile0 + tid < B) nll[tile0 + tid] = nacc;
 }
 
 // Independence Metropolis-Hastings: walker w moves to its proposal with

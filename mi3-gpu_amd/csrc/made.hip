@@ -99,36 +99,24 @@ int mi3_relu_grad(void* stream, uint64_t n, const float* dh, const float* hpre, 
     return launched();
 }
 
-int mi3_made_h1_init(void* stream, uint32_t B, uint32_t H, const float* b1, float* h1, void* h1b) {
-    if (!b1 || !h1 || !h1b) return mfail(MI3_EINVAL, "made_h1_init: bad arguments");
-    hipLaunchKernelGGL(k_made_h1_init, dim3(blocks((uint64_t)B * H)), dim3(256), 0, S(stream), B,
-                       H, b1, h1, static_cast<bf16*>(h1b));
-    return launched();
-}
-
-int mi3_made_site(void* stream, uint32_t j, const void* h2b, uint32_t H, const void* W3b,
-                  const float* b3, uint32_t q, uint32_t B, uint64_t seed, uint32_t counter,
-                  uint32_t walker_base, uint8_t* sout, uint32_t ld, float* nll) {
-    if (!h2b || !W3b || !b3 || !sout || !nll || q < 2 || q > MAX_Q || H % 8 || ld < B ||
-        !al16(h2b))
-        return mfail(MI3_EINVAL, "made_site: bad arguments");
-    const size_t lds = (size_t)q * H * sizeof(float);
-    if (lds > 160 * 1024) return mfail(MI3_EUNSUP, "made_site: q*H too large");
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_made_site),
+int mi3_made_fused(void* stream, uint32_t L, uint32_t q, uint32_t H, uint32_t B, const float* W1t,
+                   const float* b1, const void* W2b, const float* b2, const void* W3b,
+                   const float* b3, uint64_t seed, uint32_t counter, uint32_t walker_base,
+                   uint8_t* seq, uint32_t ld, float* nll, int draw) {
+    if (!W1t || !b1 || !W2b || !b2 || !W3b || !b3 || !seq || !nll || L < 2 || q < 2 ||
+        ld < B || !al16(W2b) || !al16(W3b))
+        return mfail(MI3_EINVAL, "made_fused: bad arguments");
+    if (q > (uint32_t)MF_QMAX || H == 0 || H % 16 || H > (uint32_t)MF_HMAX)
+        return mfail(MI3_EUNSUP, "made_fused: needs q <= 25 and H a multiple of 16 up to 512");
+    if (B == 0) return MI3_OK;
+    const size_t lds = sizeof(MadeFusedLds);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_made_fused),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return mfail(MI3_EHIP, hipGetErrorString(e));
-    hipLaunchKernelGGL(k_made_site, dim3(blocks(B)), dim3(256), lds, S(stream), j,
-                       static_cast<const bf16*>(h2b), H, static_cast<const bf16*>(W3b), b3, q, B,
-                       (uint32_t)seed, (uint32_t)(seed >> 32), counter, walker_base, sout, ld,
-                       nll);
-    return launched();
-}
-
-int mi3_made_h1_update(void* stream, uint32_t j, const uint8_t* sout, uint32_t ld, uint32_t B,
-                       uint32_t q, uint32_t H, const float* W1t, float* h1, void* h1b) {
-    if (!sout || !W1t || !h1 || !h1b || ld < B) return mfail(MI3_EINVAL, "made_h1_update: bad arguments");
-    hipLaunchKernelGGL(k_made_h1_update, dim3(blocks((uint64_t)B * H)), dim3(256), 0, S(stream), j,
-                       sout, ld, B, q, H, W1t, h1, static_cast<bf16*>(h1b));
+    hipLaunchKernelGGL(k_made_fused, dim3((B + MF_T - 1) / MF_T), dim3(MF_THREADS), lds, S(stream),
+                       L, q, H, B, W1t, b1, static_cast<const bf16*>(W2b), b2,
+                       static_cast<const bf16*>(W3b), b3, (uint32_t)seed, (uint32_t)(seed >> 32),
+                       counter, walker_base, seq, ld, nll, draw);
     return launched();
 }
 

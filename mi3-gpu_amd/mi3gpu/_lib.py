@@ -75,10 +75,8 @@ SIGNATURES = {
     'mi3_made_embed_grad': (_i, [_P, _P, _u32, _u32, _u32, _u32, _u32, _P, _P]),
     'mi3_made_xent': (_i, [_P, _P, _P, _u32, _u32, _u32, _u32, _f, _P, _P, _P]),
     'mi3_relu_grad': (_i, [_P, _u64, _P, _P, _P, _P]),
-    'mi3_made_h1_init': (_i, [_P, _u32, _u32, _P, _P, _P]),
-    'mi3_made_site': (_i, [_P, _u32, _P, _u32, _P, _P, _u32, _u32, _u64, _u32, _u32, _P, _u32,
-                           _P]),
-    'mi3_made_h1_update': (_i, [_P, _u32, _P, _u32, _u32, _u32, _u32, _P, _P, _P]),
+    'mi3_made_fused': (_i, [_P, _u32, _u32, _u32, _u32, _P, _P, _P, _P, _P, _P, _u64, _u32, _u32,
+                            _P, _u32, _P, _i]),
     'mi3_adam_step': (_i, [_P, _u64, _P, _P, _P, _P, _P, _f, _f, _f, _f, _u32, _P]),
     'mi3_colsum': (_i, [_P, _u32, _u32, _P, _P]),
     'mi3_to_bf16': (_i, [_P, _u64, _P, _P]),

@@ -267,24 +267,29 @@ class MADE:
         """Draw B sequences autoregressively into the site-major buffer at
         device address `out` (row stride ld); the float32 array at nll_out
         (B) receives their negative log probabilities.  Randomness from
-        Philox((site, walker_base + b, counter, 'MADE')) keyed by seed."""
-        self._check_batch(B, ld)
-        L, q, H, m = self.L, self.q, self.H, self.mem
-        st = self._st(stream)
-        _mk(lib().mi3_made_h1_init(st, B, H, m['b1'], m['h1'], m['h1b']))
-        for j in range(L):
-            self._gemm(st, 0, 1, B, H, H, m['h1b'], H, m['W2b'], H, Cb=m['h2b'], ldcb=H,
-                       bias=m['b2'], relu=1)
-            _mk(lib().mi3_made_site(st, j, m['h2b'], H, m['W3b'], m['b3'], q, B, int(seed),
-                                    int(counter) & 0xffffffff, int(walker_base), out, ld,
-                                    nll_out))
-            if j + 1 < L:
-                _mk(lib().mi3_made_h1_update(st, j, out, ld, B, q, H, m['W1t'], m['h1'],
-                                             m['h1b']))
+        Philox((site, walker_base + b, counter, 'MADE')) keyed by seed.  One
+        fused launch over all sites (each hidden unit formed once, at its
+        degree)."""
+        self._fused(B, out, ld, nll_out, seed, counter, walker_base, 1, stream)
 
     def nll(self, seq, ld, B, out, stream=None):
         """Writes -log q of B site-major sequences at `seq` to the float32
-        array at device address `out`."""
+        array at device address `out`: the sampling pass with the letters
+        read instead of drawn, so a drawn sequence scores exactly the nll
+        its draw returned."""
+        self._fused(B, seq, ld, out, 0, 0, 0, 0, stream)
+
+    def _fused(self, B, seq, ld, nll_out, seed, counter, walker_base, draw, stream):
+        self._check_batch(B, ld)
+        m = self.mem
+        _mk(lib().mi3_made_fused(self._st(stream), self.L, self.q, self.H, B, m['W1t'], m['b1'],
+                                 m['W2b'], m['b2'], m['W3b'], m['b3'], int(seed),
+                                 int(counter) & 0xffffffff, int(walker_base), seq, ld, nll_out,
+                                 int(draw)))
+
+    def nll_forward(self, seq, ld, B, out, stream=None):
+        """-log q of B sequences by the training path's forward pass (the
+        embedding, the two GEMMs and the per-sequence softmax)."""
         self._check_batch(B, ld)
         st, m = self._st(stream), self.mem
         self._forward(seq, ld, B, st)

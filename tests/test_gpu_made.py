@@ -229,3 +229,86 @@ def test_mixed_sampler_exact_and_gains(gpu_ok, tmp_path):
     assert acc > 0.2 and acc > 5 * acc_untrained
     assert ssr_m < 3 * floor
     assert ssr_p > 20 * floor
+
+
+def test_gemm_bf16_hidden_product_at_size(gpu_ok):
+    """The sampler's hidden product at its production shape (2^16 walkers x
+    512 x 512, A row-major, B transposed as W2 is stored): small integers
+    make bf16 inputs and float32 sums exact, so the result must equal numpy
+    bit for bit, as must its bf16 copy."""
+    from mi3gpu.made import DeviceArrays
+    from mi3gpu._lib import lib
+    M, N, K = 2 ** 16, 512, 512
+    rng = np.random.default_rng(17)
+    A = rng.integers(-3, 4, (M, K)).astype(np.float32)
+    Bt = rng.integers(-3, 4, (N, K)).astype(np.float32)
+    bias = rng.integers(-8, 9, N).astype(np.float32)
+    d = DeviceArrays()
+    d.alloc('A', (M, K), 'u2')
+    d.alloc('B', (N, K), 'u2')
+    d.alloc('bias', (N,), 'f4')
+    d.alloc('C', (M, N), 'f4')
+    d.alloc('Cb', (M, N), 'u2')
+    d.write('A', _to_bf16_bits(A))
+    d.write('B', _to_bf16_bits(Bt))
+    d.write('bias', bias)
+    rc = lib().mi3_gemm_bf16(d.stream, 0, 1, M, N, K, 1.0, d['A'], K, d['B'], K, 0.0,
+                             d['C'], N, d['Cb'], N, d['bias'], 1)
+    assert rc == 0
+    d.sync()
+    want = np.maximum(A @ Bt.T + bias[None, :], 0)
+    np.testing.assert_array_equal(d.read('C'), want)
+    np.testing.assert_array_equal(_from_bf16_bits(d.read('Cb')), mo.bf16(want))
+    d.close()
+
+
+def test_made_production_size_matches_oracle(gpu_ok):
+    """The learned proposal's network at its production size (L=232, q=21,
+    two hidden layers of 512) on a 256-sequence batch: per-sequence NLL and
+    every gradient against the oracle with the GPU's bf16 rounding, then the
+    sampler at that size: the NLL it accumulates while drawing equals the
+    teacher-forced NLL of what it drew."""
+    from mi3gpu.made import MADE, made_masks
+    L, q, H, B = 232, 21, 512, 256
+    m = MADE(L, q, H, max_batch=2048, seed=4)
+    _perturb(m, np.random.default_rng(8), {'W1t': 0.05, 'W2': 0.02, 'W3': 0.05, 'b3': 0.5})
+    s = np.random.default_rng(2).integers(0, q, (B, L)).astype(np.uint8)
+    m.mem.alloc('seqs', (L, B), 'u1')
+    m.mem.write('seqs', s.T)
+    m.loss_and_grads(m.mem['seqs'], B, B)
+    nll = m.get('nll')[:B].astype(np.float64)
+    P = _params(m)
+    Ms = made_masks(L, q, H)
+    g, nll_ref = mo.grads(s, *P, q, *Ms, bf=True)
+    ref_each = mo.forward(s, *P, q, bf=True)['nll']
+    rel = np.abs(nll - ref_each) / np.abs(ref_each)
+    print(f"nll mean {nll.mean():.4f} (oracle {nll_ref:.4f}), max rel {rel.max():.2e}")
+    assert rel.max() < 1e-4
+    mask = {'W1t': Ms[0], 'W2': Ms[1], 'W3': Ms[2]}
+    for name in NAMES:
+        got = m.get('g_' + name).astype(np.float64) * mask.get(name, 1.0)
+        ref = g[name]
+        err = np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-30)
+        print(f"{name}: relative error {err:.2e}")
+        assert err < 1e-3, name
+    # sampling at this size
+    n = 2048
+    m.mem.alloc('out', (L, n), 'u1')
+    m.mem.alloc('nll_s', (n,), 'f4')
+    m.mem.alloc('nll_tf', (n,), 'f4')
+    m.sample(n, m.mem['out'], n, m.mem['nll_s'], seed=5, counter=1)
+    m.nll(m.mem['out'], n, n, m.mem['nll_tf'])
+    a, b = m.get('nll_s').astype(np.float64), m.get('nll_tf').astype(np.float64)
+    drawn = m.get('out').T
+    print(f"sampled nll {a.mean():.3f}, teacher-forced {b.mean():.3f}, "
+          f"max diff {np.abs(a - b).max():.2e}")
+    assert drawn.max() < q
+    np.testing.assert_allclose(a, b, rtol=1e-4, atol=0)
+    # the drawn sequences' nll under the oracle network, and by the training
+    # path's forward pass (embedding, two GEMMs, softmax)
+    ref_drawn = mo.forward(drawn[:256], *P, q, bf=True)['nll']
+    np.testing.assert_allclose(b[:256], ref_drawn, rtol=1e-4)
+    m.mem.alloc('nll_fw', (n,), 'f4')
+    m.nll_forward(m.mem['out'], n, n, m.mem['nll_fw'])
+    np.testing.assert_allclose(m.get('nll_fw').astype(np.float64), b, rtol=1e-4)
+    m.close()
