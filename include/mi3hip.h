@@ -370,14 +370,14 @@ int mi3_made_xent(void *stream, const float *logits, const uint8_t *seq, uint32_
 int mi3_relu_grad(void *stream, uint64_t n, const float *dh, const float *hpre, float *out,
                   void *outb);
 
-/* Autoregressive pass over all L sites for B walkers (tiles of 32 per
+/* Autoregressive pass over all L sites for B walkers (tiles of 16 per
  * work-group): with draw != 0 it samples s_j (u from Philox((j,
  * walker_base+b, counter, 'MADE')) keyed by seed, by inversion) into row j
  * of the site-major `seq`; with draw == 0 it reads s_j from seq.  Either way
  * nll[b] = -sum_j log p(s_j).  Each hidden unit is formed once, at its
  * degree (the masks make it final then).  W2b, W3b are bf16 copies of the
  * masked W2 (H, H) and W3 (L*q, H); W1t (L*q, H) float32.  Needs q <= 25 and
- * H a multiple of 16 up to 512 (MI3_EUNSUP otherwise). */
+ * H a multiple of 32 up to 512 (MI3_EUNSUP otherwise). */
 int mi3_made_fused(void *stream, uint32_t L, uint32_t q, uint32_t H, uint32_t B, const float *W1t,
                    const float *b1, const void *W2b, const float *b2, const void *W3b,
                    const float *b3, uint64_t seed, uint32_t counter, uint32_t walker_base,

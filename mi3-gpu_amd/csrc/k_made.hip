@@ -222,11 +222,11 @@ __global__ void k_relu_grad(uint64_t n, const float* __restrict__ dh,
     if (outb) outb[i] = (bf16)v;
 }
 
-// Fused autoregressive pass over all L sites for tiles of MF_T = 32 walkers,
-// one work-group of 8 waves per tile: with draw != 0 it samples s_j by
-// inversion of u = Philox((j, walker_base + b, counter, TAG_MADE)) and
-// writes it to row j of seq; with draw == 0 it reads s_j from seq (teacher
-// forcing).  Either way nll[b] = -sum_j log p(s_j).
+// Fused autoregressive pass over all L sites for tiles of MF_T = 16 walkers,
+// one work-group of 4 waves per tile (several tiles share a CU): with
+// draw != 0 it samples s_j by inversion of u = Philox((j, walker_base + b,
+// counter, TAG_MADE)) and writes it to row j of seq; with draw == 0 it reads
+// s_j from seq (teacher forcing).  Either way nll[b] = -sum_j log p(s_j).
 //
 // The degree masks make each hidden unit final once: unit k (degree
 // m_k = 1 + k mod (L-1)) of layer 1 sees sites i < m_k, unit k of layer 2
@@ -238,45 +238,43 @@ __global__ void k_relu_grad(uint64_t n, const float* __restrict__ dh,
 // matrix-core work than a GEMM per site).
 //
 // Per site j: (A) layer-1 units of degree j, kept as float32 sums in
-// registers (thread t owns unit t for all 32 walkers), go to LDS as
-// bf16(relu); (B) layer-2 units of degree j: [32 walkers x H] x W2b rows of
-// those units on v_mfma_f32_32x32x16_bf16, the k-steps dealt to 4 waves
-// (wave 0 starts from the bias), the partials summed by all threads in a
-// fixed order, then bf16(relu(.)) into LDS; (C) site j's q logits the same
-// way from the layer-2 units, W3b rows q*j..q*j+q-1 and b3; (D) one thread
-// per walker: softmax, draw or read, nll, while wave 1 draws the next site's
-// uniforms; (E) the float32 layer-1 sums of units of degree > j add W1t's
-// row of the site's letter (the sums are formed in site order, as the
-// teacher-forced embedding forms them).
+// registers (thread t owns units t and t + 256 for the tile's walkers), go
+// to LDS as bf16(relu); (B) layer-2 units of degree j: [16 walkers x H] x W2b
+// rows of those units on v_mfma_f32_16x16x32_bf16, the k-steps dealt to the
+// 4 waves (wave 0 starts from the bias) and the partials summed in a fixed
+// order, then bf16(relu(.)) into LDS; (C) site j's q logits the same way
+// (two column tiles) from the layer-2 units, W3b rows q*j..q*j+q-1 and b3;
+// (D) 16 lanes per walker: softmax, draw or read, nll, then wave 1 draws the
+// next site's uniforms; (E) the float32 layer-1 sums of units of degree
+// > j add W1t's row of the site's letter (the sums are formed in site order,
+// as the teacher-forced embedding forms them).
 //
-// Everything a site reads from memory except the drawn letters is known a
-// site ahead and loaded into registers while the previous site computes:
-// the B operands and biases of (B) and (C) right after the products that
-// use them, this thread's W1t entries for every letter of the next site
-// (staged in LDS at the next site's start for (E)) and, when teacher
-// forcing, the next site's letters.  Requires H % 16 == 0, H <= 512,
-// q <= 25.
-constexpr int MF_T = 32, MF_WAVES = 8, MF_THREADS = 64 * MF_WAVES, MF_HMAX = 512;
+// The B operands and biases of (B) and (C) for site j+1 are loaded into
+// registers right after the products of site j use them; when teacher
+// forcing, so are the next site's letters.  Requires H % 32 == 0,
+// H <= 512, q <= 25.
+constexpr int MF_T = 16, MF_WAVES = 4, MF_THREADS = 64 * MF_WAVES, MF_HMAX = 512;
+constexpr int MF_HU = MF_HMAX / MF_THREADS;     // layer-1 units per thread
 constexpr int MF_QMAX = 25;
-constexpr int MF_MW = 4;                        // waves that run the products
 constexpr int MF_HP = MF_HMAX + 8;              // padded LDS row (bf16) of a walker's units
-constexpr int MF_KS = MF_HMAX / 16 / MF_MW;     // k-steps of 16 per product wave
-static_assert(MF_THREADS == MF_HMAX, "one layer-1 unit per thread");
+constexpr int MF_KS = MF_HMAX / 32 / MF_WAVES;  // k-steps of 32 per wave
+constexpr int MF_NC = 32;                       // logit columns (two tiles of 16)
+static_assert(MF_THREADS == 16 * MF_T, "16 lanes per walker in the softmax");
 
 struct MadeFusedLds {
     bf16 h1[MF_T][MF_HP];                  // bf16(relu(layer 1)), final units only
     bf16 h2[MF_T][MF_HP];                  // bf16(relu(layer 2)), final units only
-    float part[MF_MW][MF_T][MF_T + 1];     // per-wave partial products
-    float z[MF_T][MF_T + 1];               // the site's logits
-    float w1[MF_QMAX][MF_HMAX];            // W1t entries of the site by letter
+    float part[MF_WAVES][MF_T][MF_NC + 1]; // per-wave partial products
+    float z[MF_T][MF_NC + 1];              // the site's logits
     float u[2][MF_T];                      // uniforms of this / the next site
     uint32_t s[MF_T];                      // the site's letters
 };
 
-// Operands of one product wave's share of a [32 x H] x [H x 32] product: the
-// B operand (this lane's 8 k-values of column lane%32 for each of the wave's
-// k-steps; zero for a negative row or k past H) and, for wave 0, the bias of
-// the column (the accumulator's start).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// B operand of one wave's k-steps for one 16-column tile: this lane's 8
+// k-values of column lane%16 (zero for a negative row or k past H), and for
+// wave 0 the column's bias (the accumulator's start).
 struct MadeB {
     bf16x8 b[MF_KS];
     float bias;
@@ -288,33 +286,34 @@ __device__ __forceinline__ void made_load_b(MadeB& r, const bf16* __restrict__ W
     const bf16x8 zero8 = {};
 #pragma unroll
     for (int t = 0; t < MF_KS; t++) {
-        const uint32_t k0 = 16 * (wave + MF_MW * t);
+        const uint32_t k0 = 32 * (wave + MF_WAVES * t);
         r.b[t] = (row >= 0 && k0 < H)
             ? *reinterpret_cast<const bf16x8*>(W + (size_t)row * H + k0 + 8 * lg) : zero8;
     }
     r.bias = (wave == 0 && row >= 0) ? bias[row] : 0.f;
 }
 
-// part[wave] = bias (wave 0) + X[32 x H] . (this wave's k-steps of b)
+// part[wave][.][c0 + .] = bias (wave 0) + X[16 x H] . (this wave's k-steps of b)
 __device__ __forceinline__ void made_product(const bf16 (*X)[MF_HP], const MadeB& b, uint32_t H,
-                                             float (*part)[MF_T + 1], int lr, int lg, int wave) {
-    f32x16 acc;
+                                             float (*part)[MF_NC + 1], int c0, int lr, int lg,
+                                             int wave) {
+    f32x4 acc;
 #pragma unroll
-    for (int e = 0; e < 16; e++) acc[e] = b.bias;
+    for (int e = 0; e < 4; e++) acc[e] = b.bias;
 #pragma unroll
     for (int t = 0; t < MF_KS; t++) {
-        const uint32_t k0 = 16 * (wave + MF_MW * t);
+        const uint32_t k0 = 32 * (wave + MF_WAVES * t);
         if (k0 < H) {
             const bf16x8 a = *reinterpret_cast<const bf16x8*>(&X[lr][k0 + 8 * lg]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b.b[t], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b.b[t], acc, 0, 0, 0);
         }
     }
-    // D layout: value e of lane l is row 8*(e/4) + 4*(l/32) + e%4, column l%32
+    // D layout: value e of lane l is row 4*(l/16) + e, column l%16
 #pragma unroll
-    for (int e = 0; e < 16; e++) part[8 * (e >> 2) + 4 * lg + (e & 3)][lr] = acc[e];
+    for (int e = 0; e < 4; e++) part[4 * lg + e][c0 + lr] = acc[e];
 }
 
-__device__ __forceinline__ float made_part_sum(const float (*part)[MF_T][MF_T + 1], uint32_t r,
+__device__ __forceinline__ float made_part_sum(const float (*part)[MF_T][MF_NC + 1], uint32_t r,
                                                uint32_t c) {
     return ((part[0][r][c] + part[1][r][c]) + part[2][r][c]) + part[3][r][c];
 }
@@ -325,51 +324,44 @@ __global__ __launch_bounds__(MF_THREADS) void k_made_fused(
     const bf16* __restrict__ W3b, const float* __restrict__ b3, uint32_t k0, uint32_t k1,
     uint32_t counter, uint32_t walker_base, uint8_t* __restrict__ seq, uint32_t ld,
     float* __restrict__ nll, int draw) {
-    extern __shared__ __attribute__((aligned(16))) char made_lds_raw[];
-    MadeFusedLds& sm = *reinterpret_cast<MadeFusedLds*>(made_lds_raw);
+    __shared__ MadeFusedLds sm;
     const int tid = threadIdx.x, lane = tid & 63;
-    const int lr = lane & 31, lg = lane >> 5;
+    const int lr = lane & 15, lg = lane >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const bool mwave = wave < MF_MW;
     const uint32_t tile0 = blockIdx.x * MF_T;
     const uint32_t Lm1 = L - 1;
     // units of degree jj: k = (jj - 1) + c (L - 1), c = 0, 1, ...
     auto n_units = [&](uint32_t jj) -> uint32_t {
         return (jj >= 1 && jj < L && jj - 1 < H) ? (H - (jj - 1) + Lm1 - 1) / Lm1 : 0u;
     };
-    const uint32_t k = tid;                       // this thread's layer-1 unit
-    const uint32_t deg = k < H ? 1 + k % Lm1 : 0xffffffffu;
-    float h1r[MF_T];
-    {
+    uint32_t deg[MF_HU];
+    float h1r[MF_HU][MF_T];
+#pragma unroll
+    for (int u = 0; u < MF_HU; u++) {
+        const uint32_t k = tid + MF_THREADS * u;
+        deg[u] = k < H ? 1 + k % Lm1 : 0xffffffffu;
         const float v = k < H ? b1[k] : 0.f;
 #pragma unroll
-        for (int w = 0; w < MF_T; w++) h1r[w] = v;
+        for (int w = 0; w < MF_T; w++) h1r[u][w] = v;
     }
     for (uint32_t e = tid; e < MF_T * MF_HP; e += MF_THREADS) {
         (&sm.h1[0][0])[e] = (bf16)0.f;
         (&sm.h2[0][0])[e] = (bf16)0.f;
     }
-    // this thread's W1t entries for every letter of site jj (zero where the
-    // unit does not see the site)
-    float w1r[MF_QMAX];
-    auto load_w1 = [&](uint32_t jj) {
-        const bool sees = deg != 0xffffffffu && deg > jj && jj < L;
-#pragma unroll
-        for (uint32_t a = 0; a < MF_QMAX; a++)
-            w1r[a] = (sees && a < q) ? W1t[(size_t)(q * jj + a) * H + k] : 0.f;
-    };
-    MadeB w2r, w3r;
+    MadeB w2r, w3r[2];
     auto load_w2 = [&](uint32_t jj) {
         const uint32_t nU = n_units(jj);
-        const int row = (mwave && nU <= MF_T && (uint32_t)lr < nU) ? (int)((jj - 1) + lr * Lm1)
-                                                                   : -1;
+        const int row = (nU <= MF_T && (uint32_t)lr < nU) ? (int)((jj - 1) + lr * Lm1) : -1;
         made_load_b(w2r, W2b, b2, H, row, lg, wave);
     };
     auto load_w3 = [&](uint32_t jj) {
-        const int row = (mwave && jj < L && (uint32_t)lr < q) ? (int)(q * jj + lr) : -1;
-        made_load_b(w3r, W3b, b3, H, row, lg, wave);
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const uint32_t c = 16 * t + lr;
+            made_load_b(w3r[t], W3b, b3, H, (jj < L && c < q) ? (int)(q * jj + c) : -1, lg, wave);
+        }
     };
-    // uniforms of site jj for the tile's walkers (wave 1, lanes < 32)
+    // uniforms of site jj for the tile's walkers (wave 1, lanes < 16)
     auto draw_u = [&](uint32_t jj) {
         if (draw && wave == 1 && lane < MF_T && jj < L) {
             const uint4 r = philox4x32_10(make_uint4(jj, walker_base + tile0 + lane, counter,
@@ -377,42 +369,41 @@ __global__ __launch_bounds__(MF_THREADS) void k_made_fused(
             sm.u[jj & 1][lane] = (float)(r.x >> 8) * 0x1.0p-24f;
         }
     };
-    // teacher forcing: thread w < 32 holds the letter of walker w at the next site
+    // teacher forcing: every lane of walker w's group holds its letter at the
+    // next site
     uint32_t s_next = 0;
     auto load_s = [&](uint32_t jj) {
-        const uint32_t b = tile0 + tid;
-        s_next = (!draw && tid < MF_T && b < B && jj < L) ? seq[(size_t)jj * ld + b] : 0u;
+        const uint32_t b = tile0 + ((uint32_t)tid >> 4);
+        s_next = (!draw && b < B && jj < L) ? seq[(size_t)jj * ld + b] : 0u;
     };
     load_w2(0);
     load_w3(0);
-    load_w1(0);
     load_s(0);
     draw_u(0);
-    float nacc = 0.f;    // thread w < 32: nll of walker tile0 + w
+    float nacc = 0.f;    // nll of walker tile0 + tid / 16 (every lane of its group)
 
     for (uint32_t j = 0; j < L; j++) {
-        // (A) layer-1 units of degree j are final; the site's W1t entries
+        // (A) layer-1 units of degree j are final
 #pragma unroll
-        for (uint32_t a = 0; a < MF_QMAX; a++) sm.w1[a][k] = w1r[a];
-        if (deg == j) {
+        for (int u = 0; u < MF_HU; u++) {
+            if (deg[u] == j) {
+                const uint32_t k = tid + MF_THREADS * u;
 #pragma unroll
-            for (int w = 0; w < MF_T; w++) sm.h1[w][k] = (bf16)fmaxf(h1r[w], 0.f);
+                for (int w = 0; w < MF_T; w++) sm.h1[w][k] = (bf16)fmaxf(h1r[u][w], 0.f);
+            }
         }
         __syncthreads();
-        // (B) layer-2 units of degree j
+        // (B) layer-2 units of degree j, 16 at a time
         const uint32_t nU = n_units(j);
         for (uint32_t c0 = 0; c0 < nU; c0 += MF_T) {
-            if (mwave) {
-                if (nU > MF_T) {   // more than one column tile: operands loaded here
-                    const uint32_t c = c0 + lr;
-                    made_load_b(w2r, W2b, b2, H, c < nU ? (int)((j - 1) + c * Lm1) : -1, lg,
-                                wave);
-                }
-                made_product(sm.h1, w2r, H, sm.part[wave], lr, lg, wave);
+            if (nU > MF_T) {   // more than one column tile: operands loaded here
+                const uint32_t c = c0 + lr;
+                made_load_b(w2r, W2b, b2, H, c < nU ? (int)((j - 1) + c * Lm1) : -1, lg, wave);
             }
+            made_product(sm.h1, w2r, H, sm.part[wave], 0, lr, lg, wave);
             __syncthreads();
-            for (uint32_t e = tid; e < MF_T * MF_T; e += MF_THREADS) {
-                const uint32_t r = e / MF_T, c = e % MF_T;
+            {
+                const uint32_t r = tid / MF_T, c = tid % MF_T;   // 256 = 16 x 16 entries
                 if (c0 + c < nU)
                     sm.h2[r][(j - 1) + (c0 + c) * Lm1] =
                         (bf16)fmaxf(made_part_sum(sm.part, r, c), 0.f);
@@ -421,64 +412,80 @@ __global__ __launch_bounds__(MF_THREADS) void k_made_fused(
         }
         load_w2(j + 1);
         // (C) logits of site j (columns a < q), b3 included
-        if (mwave) made_product(sm.h2, w3r, H, sm.part[wave], lr, lg, wave);
+        made_product(sm.h2, w3r[0], H, sm.part[wave], 0, lr, lg, wave);
+        made_product(sm.h2, w3r[1], H, sm.part[wave], 16, lr, lg, wave);
         load_w3(j + 1);
         __syncthreads();
-        for (uint32_t e = tid; e < MF_T * MF_T; e += MF_THREADS) {
-            const uint32_t r = e / MF_T, c = e % MF_T;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const uint32_t e = tid + MF_THREADS * h, r = e / MF_NC, c = e % MF_NC;
             if (c < q) sm.z[r][c] = made_part_sum(sm.part, r, c);
         }
         __syncthreads();
-        // (D) softmax and the letter, one thread per walker; wave 1 draws
-        // the next site's uniforms meanwhile
-        if (tid < MF_T) {
-            const uint32_t b = tile0 + tid;
-            float ez[MF_QMAX];
-            float mx = -INFINITY;
+        // (D) softmax and the letter: 16 lanes per walker (walker tid / 16;
+        // lane l holds letters l and l + 16), max, sum and the inverse-CDF
+        // search by 16-lane shuffles and one ballot; wave 1 then draws the
+        // next site's uniforms
+        {
+            const uint32_t w = (uint32_t)tid >> 4, l = (uint32_t)tid & 15;
+            const float z1 = (l < q) ? sm.z[w][l] : -INFINITY;
+            const float z2 = (l + 16 < q) ? sm.z[w][l + 16] : -INFINITY;
+            float mx = fmaxf(z1, z2);
 #pragma unroll
-            for (uint32_t a = 0; a < MF_QMAX; a++)
-                if (a < q) mx = fmaxf(mx, sm.z[tid][a]);
-            float se = 0.f;
+            for (int o = 8; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 16));
+            float c1 = (l < q) ? __expf(z1 - mx) : 0.f;
+            float c2 = (l + 16 < q) ? __expf(z2 - mx) : 0.f;
+            // inclusive prefix sums over the group (letters in order)
 #pragma unroll
-            for (uint32_t a = 0; a < MF_QMAX; a++) {
-                ez[a] = (a < q) ? expf(sm.z[tid][a] - mx) : 0.f;
-                se += ez[a];
-            }
-            uint32_t s = 0;
-            if (draw) {
-                const float uu = sm.u[j & 1][tid] * se;
-                float cm = 0.f;
-                s = q - 1;
-                bool found = false;
-#pragma unroll
-                for (uint32_t a = 0; a < MF_QMAX; a++) {
-                    if (a + 1 < q && !found) {
-                        cm += ez[a];
-                        if (uu < cm) {
-                            s = a;
-                            found = true;
-                        }
-                    }
+            for (int o = 1; o < 16; o <<= 1) {
+                const float t1 = __shfl_up(c1, o, 16), t2 = __shfl_up(c2, o, 16);
+                if (l >= (uint32_t)o) {
+                    c1 += t1;
+                    c2 += t2;
                 }
-                if (b < B) seq[(size_t)j * ld + b] = (uint8_t)s;
+            }
+            const float tot1 = __shfl(c1, 15, 16);
+            const float se = tot1 + __shfl(c2, 15, 16);
+            uint32_t s;
+            if (draw) {
+                // first letter a (a + 1 < q) whose cumulative mass exceeds u
+                const float uu = sm.u[j & 1][w] * se;
+                const uint64_t m1 = __ballot((l + 1 < q) && uu < c1);
+                const uint64_t m2 = __ballot((l + 17 < q) && uu < tot1 + c2);
+                const uint32_t sh = (uint32_t)tid & 48u;
+                const uint32_t g1 = (uint32_t)(m1 >> sh) & 0xffffu;
+                const uint32_t g2 = (uint32_t)(m2 >> sh) & 0xffffu;
+                s = g1 ? (uint32_t)__builtin_ctz(g1)
+                       : (g2 ? 16u + (uint32_t)__builtin_ctz(g2) : q - 1);
+                if (l == 0 && tile0 + w < B) seq[(size_t)j * ld + tile0 + w] = (uint8_t)s;
             } else {
                 s = s_next < q ? s_next : q - 1;
             }
-            nacc += mx + logf(se) - sm.z[tid][s];
-            sm.s[tid] = s;
+            nacc += mx + __logf(se) - sm.z[w][s];
+            if (l == 0) sm.s[w] = s;
         }
         draw_u(j + 1);
         load_s(j + 1);
         __syncthreads();
-        // (E) sums of the layer-1 units that see site j (units that do not
-        // see it read a zero entry)
+        // (E) sums of the layer-1 units that see site j
         if (j + 1 < L) {
+            uint32_t sw[MF_T];
 #pragma unroll
-            for (int w = 0; w < MF_T; w++) h1r[w] += sm.w1[sm.s[w]][k];
-            load_w1(j + 1);
+            for (int w = 0; w < MF_T; w++) sw[w] = sm.s[w];
+#pragma unroll
+            for (int u = 0; u < MF_HU; u++) {
+                if (deg[u] != 0xffffffffu && deg[u] > j) {
+                    const uint32_t k = tid + MF_THREADS * u;
+                    float v[MF_T];
+#pragma unroll
+                    for (int w = 0; w < MF_T; w++) v[w] = W1t[(size_t)(q * j + sw[w]) * H + k];
+#pragma unroll
+                    for (int w = 0; w < MF_T; w++) h1r[u][w] += v[w];
+                }
+            }
         }
     }
-    if (tid < MF_T && tile0 + tid < B) nll[tile0 + tid] = nacc;
+    if ((tid & 15) == 0 && tile0 + (tid >> 4) < B) nll[tile0 + (tid >> 4)] = nacc;
 }
 
 // Independence Metropolis-Hastings: walker w moves to its proposal with

@@ -106,14 +106,10 @@ int mi3_made_fused(void* stream, uint32_t L, uint32_t q, uint32_t H, uint32_t B,
     if (!W1t || !b1 || !W2b || !b2 || !W3b || !b3 || !seq || !nll || L < 2 || q < 2 ||
         ld < B || !al16(W2b) || !al16(W3b))
         return mfail(MI3_EINVAL, "made_fused: bad arguments");
-    if (q > (uint32_t)MF_QMAX || H == 0 || H % 16 || H > (uint32_t)MF_HMAX)
-        return mfail(MI3_EUNSUP, "made_fused: needs q <= 25 and H a multiple of 16 up to 512");
+    if (q > (uint32_t)MF_QMAX || H == 0 || H % 32 || H > (uint32_t)MF_HMAX)
+        return mfail(MI3_EUNSUP, "made_fused: needs q <= 25 and H a multiple of 32 up to 512");
     if (B == 0) return MI3_OK;
-    const size_t lds = sizeof(MadeFusedLds);
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_made_fused),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return mfail(MI3_EHIP, hipGetErrorString(e));
-    hipLaunchKernelGGL(k_made_fused, dim3((B + MF_T - 1) / MF_T), dim3(MF_THREADS), lds, S(stream),
+    hipLaunchKernelGGL(k_made_fused, dim3((B + MF_T - 1) / MF_T), dim3(MF_THREADS), 0, S(stream),
                        L, q, H, B, W1t, b1, static_cast<const bf16*>(W2b), b2,
                        static_cast<const bf16*>(W3b), b3, (uint32_t)seed, (uint32_t)(seed >> 32),
                        counter, walker_base, seq, ld, nll, draw);
