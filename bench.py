@@ -57,7 +57,7 @@ MEASURED_TRAFFIC = {(232, 21, 2 ** 18, 8, 512): 13.41e9}
 # VALU issue of the same k_sweep launch from SQ counters
 # (SQ_ACTIVE_INST_VALU per CU over SQ_BUSY_CYCLES; summary and command in
 # profiles/k_sweep_L232_q21_N262144_kernel_trace.md).  Same key.
-MEASURED_VALU_BUSY = {(232, 21, 2 ** 18, 8, 512): 0.71}
+MEASURED_VALU_BUSY = {(232, 21, 2 ** 18, 8, 512): 0.69}
 
 
 def host_cores():
@@ -303,10 +303,12 @@ def measure_kernels(n=2 ** 20, L=232, q=21, reps=3):
         g.calcEnergies('main')
         g.wait()
         g.profile_reset()
-        for _ in range(reps):
+        for r in range(reps):
             g.calcBicounts('main')
             g.calcEnergies('main')
-        g.wait()
+            g.wait()
+            print(f"kernels: bicounts + energies, repetition {r + 1} of {reps}",
+                  file=sys.stderr, flush=True)
         bc_ms, _ = g.kernel_ms('calcBicounts')
         en_ms, _ = g.kernel_ms('getEnergies')
         bc = g.getBuf('bicount').read()
@@ -319,6 +321,54 @@ def measure_kernels(n=2 ** 20, L=232, q=21, reps=3):
             "bicounts_frac_of_5.0_POPS": 2 * macs / (bc_ms * 1e-3) / 5.0e15,
             "bicount_rows_sum_to_n": ok_rows,
             "energies_ms": en_ms, "energy_gathers_per_s": float(n) * nP / (en_ms * 1e-3)}
+
+
+def measure_newton_kernels(n=2 ** 18, L=232, q=21, reps=5, seed=9):
+    """One quasi-Newton iteration's kernels at the production walker count
+    on equal-weight-ish Zwanzig weights: energies under a dJ, weights, and
+    weighted marginals, average kernel time (HIP events).  Rates: energies
+    in coupling gathers (n*L(L-1)/2 per call) and L2->LDS staging bytes;
+    weighted marginals in 64-bit LDS bin additions (n*L(L-1)/2 per call)."""
+    from mi3gpu.mcmcGPU import MCMCGPU
+    rng = np.random.default_rng(seed)
+    nP = L * (L - 1) // 2
+    J = production_model(L, q, seed=1)
+    dJ = rng.normal(0, 1e-3, size=(nP, q * q)).astype('<f4')
+    seqs = rng.integers(0, q, size=(n, L)).astype('u1')
+    with tempfile.TemporaryDirectory() as d:
+        g = MCMCGPU((0, 0, None, None), L, q, n, 512, Path(d), 1024, 2, profile=True)
+        g.initJstep()
+        g.setBuf('J', J)
+        g.setBuf('dJ', dJ)
+        g.setBuf('seq main', seqs)
+
+        def iteration():
+            g.calcEnergies('main', 'dJ')
+            g.min_buf('E main')
+            g.dE_to_weights('main', 0.0)
+            g.weight_statistics('main')
+            g.weightedMarg('main')
+
+        iteration()
+        g.wait()
+        g.profile_reset()
+        for r in range(reps):
+            iteration()
+            g.wait()
+            print(f"kernels: quasi-Newton iteration {r + 1} of {reps}", file=sys.stderr,
+                  flush=True)
+        en_ms, _ = g.kernel_ms('getEnergies')
+        wm_ms, _ = g.kernel_ms('weightedMarg')
+        del g
+    units = sum(max(0, (L + 31) // 32 - (i + 1) // 32) for i in range(L - 1))
+    chunk_bytes = _pf_class(q) * 512 * 16
+    stage = (n // 512) * units * chunk_bytes
+    return {"sequences": n, "L": L, "q": q,
+            "energies_ms": en_ms, "energy_gathers_per_s": float(n) * nP / (en_ms * 1e-3),
+            "energies_staging_bytes": stage,
+            "energies_staging_TBps": stage / (en_ms * 1e-3) / 1e12,
+            "weighted_marg_ms": wm_ms,
+            "weighted_marg_u64_adds_per_s": float(n) * nP / (wm_ms * 1e-3)}
 
 
 def _best_wgsize(J, s0, L, q, seed):
@@ -432,8 +482,14 @@ def measure_made(J, seqs, L, q, ntrain=2 ** 18, N=2 ** 16, H=512, train_steps=15
     print(f"mixed: tau {mixed['tau_int']:.2f} cycles, acceptance {mixed['acceptance']:.3f}",
           file=sys.stderr, flush=True)
     made.close()
+    # the mixed form's ceiling at this proposal cost: with every proposal
+    # accepted the energy decorrelates each cycle (tau = 1/2 cycle), so
+    # energy-ESS/s = N / t_cycle against the plain N / (2 tau t_sweep)
+    t_sweep, t_prop = plain["interval_ms"], mixed["proposal_step_ms"]
+    bound = 2 * plain["tau_int"] * t_sweep / (ratio * t_sweep + t_prop)
     return {"value": mixed["energy_ess_per_s"], "unit": "energy effective samples/s",
             "vs_plain": mixed["energy_ess_per_s"] / plain["energy_ess_per_s"],
+            "vs_plain_bound_all_accepted": bound,
             "plain": dict(plain, interval="1 sweep"),
             "mixed": dict(mixed, interval=f"{ratio} sweeps + 1 MADE proposal per walker"),
             "walkers": N, "walkers_per_workgroup": wg,
@@ -572,13 +628,15 @@ def main():
                     help="equiltime of the timed rounds (launches of 16 sweeps, or 'auto')")
     ap.add_argument('--round-nsteps', type=int, default=0,
                     help="nsteps of the timed rounds (default 16*L)")
-    ap.add_argument('--hiv-rounds', type=int, default=0,
-                    help="also run the reference's HIV example (pbs.sh settings) for this "
-                         "many rounds and compare with its published convergence")
+    ap.add_argument('--hiv-rounds', type=int, default=64,
+                    help="rounds of the reference's HIV example (pbs.sh settings, 64 in the "
+                         "reference) compared with its published convergence; it gives the "
+                         "s per round at fixed SSR (0 skips it)")
     ap.add_argument('--no-config1', action='store_true',
                     help="skip the L=99 q=4 round on the GPU and its CPU reference path")
     ap.add_argument('--kernels', action='store_true',
-                    help="also time bicounts and energies standalone on 2^20 sequences")
+                    help="also time bicounts and energies standalone on 2^20 sequences, "
+                         "and one quasi-Newton iteration's kernels on 2^18")
     ap.add_argument('--batch-rounds', type=int, default=0,
                     help="also time the batched uneven-models workload (rounds)")
     ap.add_argument('--made', action='store_true',
@@ -698,8 +756,17 @@ def main():
                                           a.round_nsteps or None)
     if a.kernels:
         out["kernels"] = measure_kernels()
+        out["newton_kernels"] = measure_newton_kernels()
     if a.hiv_rounds > 0:
-        out["hiv_reference_run"] = measure_hiv(a.hiv_rounds)
+        hiv = measure_hiv(a.hiv_rounds)
+        out["hiv_reference_run"] = hiv
+        # the third metric: wall time of the first round within twice the
+        # SSR floor, on the reference's own published run
+        fx = hiv["fixed_ssr"]
+        out["s_per_round_at_fixed_ssr"] = None if fx is None else {
+            "value": fx["s"], "unit": "s/round", "round": fx["round"], "ssr": fx["ssr"],
+            "ssr_floor": hiv["ssr_floor"], "cumulative_s": fx["cumulative_s"],
+            "workload": hiv["workload"]}
     if not a.no_config1 and not a.no_cpu:
         out["config1_round"] = measure_config1()
     if a.batch_rounds > 0:
