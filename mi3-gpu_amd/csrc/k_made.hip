@@ -142,33 +142,46 @@ __global__ void k_made_embed(const uint8_t* __restrict__ seq, uint32_t ld, uint3
 }
 
 // dW1t[(q*i + a)*H + k] = sum over b with s_i(b) == a of dh1[b][k]: one
-// thread per (site i, unit k) holds the q sums in registers and walks the
-// batch (the site's residues are read as a broadcast, dh1 coalesced along
-// k).  Writes every entry, so no zeroing is needed; deterministic.
-__global__ __launch_bounds__(256) void k_made_embed_grad(const uint8_t* __restrict__ seq,
-                                                         uint32_t ld, uint32_t B, uint32_t L,
-                                                         uint32_t q, uint32_t H,
-                                                         const float* __restrict__ dh1,
-                                                         float* __restrict__ dW1t) {
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint64_t)L * H) return;
-    const uint32_t i = (uint32_t)(idx / H), k = (uint32_t)(idx - (uint64_t)i * H);
-    float acc[32];
-#pragma unroll
-    for (uint32_t a = 0; a < 32; a++) acc[a] = 0.f;
+// work-group per (site i, 256 units); thread k keeps its q sums in an LDS
+// column of its own and walks the batch in order, adding dh1[b][k] into the
+// row of the letter s_i(b) (a wave-uniform scalar load), so each sequence
+// costs one LDS read and write instead of a select over every letter.
+// Writes every entry, so no zeroing is needed; deterministic.
+constexpr int EG_T = 256, EG_QMAX = 25;
+__global__ __launch_bounds__(EG_T) void k_made_embed_grad(const uint8_t* __restrict__ seq,
+                                                          uint32_t ld, uint32_t B, uint32_t L,
+                                                          uint32_t q, uint32_t H,
+                                                          const float* __restrict__ dh1,
+                                                          float* __restrict__ dW1t) {
+    __shared__ float acc[EG_QMAX][EG_T];
+    const uint32_t nkb = (H + EG_T - 1) / EG_T;
+    const uint32_t i = blockIdx.x / nkb, k = (blockIdx.x % nkb) * EG_T + threadIdx.x;
+    for (uint32_t a = 0; a < q; a++) acc[a][threadIdx.x] = 0.f;
     const uint8_t* si = seq + (size_t)i * ld;
-    for (uint32_t b = 0; b < B; b++) {
-        const uint32_t s = si[b];
-        const float g = dh1[(size_t)b * H + k];
+    if (k < H) {
+        // eight sequences per step: their letters and gradients are loaded
+        // before the eight additions (in batch order)
+        constexpr int U = 8;
+        uint32_t b = 0;
+        for (; b + U <= B; b += U) {
+            uint32_t sv[U];
+            float gv[U];
 #pragma unroll
-        for (uint32_t a = 0; a < 32; a++) acc[a] += (a == s) ? g : 0.f;
-    }
-    for (uint32_t a = 0; a < q; a++) {
-        float v = 0.f;
+            for (int u = 0; u < U; u++) {
+                sv[u] = si[b + u];
+                gv[u] = dh1[(size_t)(b + u) * H + k];
+            }
 #pragma unroll
-        for (uint32_t e = 0; e < 32; e++)
-            if (e == a) v = acc[e];
-        dW1t[(size_t)(q * i + a) * H + k] = v;
+            for (int u = 0; u < U; u++) {
+                const uint32_t s = __builtin_amdgcn_readfirstlane(sv[u]);
+                acc[s < q ? s : q - 1][threadIdx.x] += gv[u];
+            }
+        }
+        for (; b < B; b++) {
+            const uint32_t s = __builtin_amdgcn_readfirstlane(si[b]);
+            acc[s < q ? s : q - 1][threadIdx.x] += dh1[(size_t)b * H + k];
+        }
+        for (uint32_t a = 0; a < q; a++) dW1t[(size_t)(q * i + a) * H + k] = acc[a][threadIdx.x];
     }
 }
 
@@ -535,15 +548,29 @@ __global__ void k_adam(uint64_t n, float* __restrict__ p, const float* __restric
     if (pb) pb[i] = (bf16)pi;
 }
 
-// out[c] = sum over rows r of x[r][c] (rows x cols row-major), one thread
-// per column, rows in order (deterministic).
-__global__ void k_colsum(uint32_t rows, uint32_t cols, const float* __restrict__ x,
-                         float* __restrict__ out) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// out[c] = sum over rows r of x[r][c] (rows x cols row-major).  A work-group
+// takes 32 columns with 8 row lanes each: row lane t sums rows t, t + 8, ...
+// in order (each load instruction reads 8 rows x 128 contiguous bytes), then
+// the 8 partial sums are added in a fixed order, so the result is
+// deterministic.
+constexpr int CS_COLS = 32, CS_ROWL = 8;
+__global__ __launch_bounds__(CS_COLS * CS_ROWL) void k_colsum(uint32_t rows, uint32_t cols,
+                                                              const float* __restrict__ x,
+                                                              float* __restrict__ out) {
+    __shared__ float part[CS_ROWL][CS_COLS + 1];
+    const uint32_t cl = threadIdx.x % CS_COLS, t = threadIdx.x / CS_COLS;
+    const uint32_t c = blockIdx.x * CS_COLS + cl;
     float s = 0.f;
-    for (uint32_t r = 0; r < rows; r++) s += x[(size_t)r * cols + c];
-    out[c] = s;
+    if (c < cols)
+        for (uint32_t r = t; r < rows; r += CS_ROWL) s += x[(size_t)r * cols + c];
+    part[t][cl] = s;
+    __syncthreads();
+    if (t == 0 && c < cols) {
+        float v = part[0][cl];
+#pragma unroll
+        for (int k = 1; k < CS_ROWL; k++) v += part[k][cl];
+        out[c] = v;
+    }
 }
 
 // pb = bf16(p).

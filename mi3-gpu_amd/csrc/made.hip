@@ -72,12 +72,15 @@ int mi3_made_embed(void* stream, const uint8_t* seq, uint32_t ld, uint32_t B, ui
     return launched();
 }
 
+static_assert(MAX_Q <= (uint32_t)EG_QMAX, "letter sums of the embedding gradient");
+
 int mi3_made_embed_grad(void* stream, const uint8_t* seq, uint32_t ld, uint32_t B, uint32_t L,
                         uint32_t q, uint32_t H, const float* dh1, float* dW1t) {
     if (!seq || !dh1 || !dW1t || ld < B || q < 2 || q > MAX_Q)
         return mfail(MI3_EINVAL, "made_embed_grad: bad arguments");
-    hipLaunchKernelGGL(k_made_embed_grad, dim3(blocks((uint64_t)L * H)), dim3(256), 0, S(stream),
-                       seq, ld, B, L, q, H, dh1, dW1t);
+    if (L == 0 || H == 0) return MI3_OK;
+    hipLaunchKernelGGL(k_made_embed_grad, dim3(L * ((H + EG_T - 1) / EG_T)), dim3(EG_T), 0,
+                       S(stream), seq, ld, B, L, q, H, dh1, dW1t);
     return launched();
 }
 
@@ -141,7 +144,9 @@ int mi3_adam_step(void* stream, uint64_t n, float* p, const float* g, float* m, 
 
 int mi3_colsum(void* stream, uint32_t rows, uint32_t cols, const float* x, float* out) {
     if (!x || !out) return mfail(MI3_EINVAL, "colsum: bad arguments");
-    hipLaunchKernelGGL(k_colsum, dim3(blocks(cols)), dim3(256), 0, S(stream), rows, cols, x, out);
+    if (cols == 0) return MI3_OK;
+    hipLaunchKernelGGL(k_colsum, dim3((cols + CS_COLS - 1) / CS_COLS), dim3(CS_COLS * CS_ROWL), 0,
+                       S(stream), rows, cols, x, out);
     return launched();
 }
 
