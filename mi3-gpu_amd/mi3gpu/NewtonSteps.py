@@ -409,7 +409,14 @@ def MCMCstep(runName, Jstep, couplings, param, gpus, log):
                                   'jstep': int(Jstep), 'equil_launches': int(equilsteps),
                                   'ssr': float(ssr), 'ferr': float(ferr), 'X': float(X),
                                   'Xo': float(Xo)})
+        if round_hook is not None:
+            round_hook(param.round_stats[-1])
     return Jstep + Jsteps, seqs, energies, newJ
+
+
+# Optional callable given each round's timing record as it completes (a
+# caller that runs many rounds, such as a benchmark, can report progress).
+round_hook = None
 
 
 def newtonMCMC(param, gpus, start_run, Jstep, log):
