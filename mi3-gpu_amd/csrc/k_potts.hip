@@ -170,6 +170,26 @@ __device__ __forceinline__ void chunk_dma(__amdgpu_buffer_rsrc_t r, uint32_t vof
 // vector-memory operation of this wave, including LDS DMA, has completed.
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// s_waitcnt vmcnt(N): at most N vector-memory operations of this wave are
+// still in flight (gfx9 encoding: vmcnt bits 3:0 and 15:14).  Loads complete
+// in issue order, so all but the youngest N have landed.
+template <int N>
+__device__ __forceinline__ void wait_vmem_upto() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 15) | ((N >> 4) << 14));
+}
+
+// Work-group barrier that leaves LDS DMA in flight.  __syncthreads() waits
+// for vmcnt(0) (its workgroup release fence covers LDS DMA writes), which
+// drains every copy issued ahead; here only this wave's LDS and scalar loads
+// are awaited (lgkmcnt(0) as a builtin, so that the compiler's own wait
+// tracking sees them complete), then s_barrier, whose memory clobber keeps
+// memory accesses on their side.
+__device__ __forceinline__ void barrier_copies_in_flight() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    asm volatile("s_barrier" ::: "memory");
+}
+
 // Value of `v` in lane (src + 32*half) of each half-wave, src a compile-time
 // lane number < 32 (ds_swizzle bitmask mode: and 0, or src).
 #define MI3_SWZ_CASE(n) \
@@ -512,8 +532,10 @@ k_sweep_batch(const BatchModel* __restrict__ models, const BatchTile* __restrict
 // and instructions, one barrier per chunk) with no gathers, reduction or
 // accept.  With INFLIGHT = 1 each chunk's copy is awaited before the barrier
 // that follows its issue, as in the sweep; with INFLIGHT = 2 the copy of the
-// next chunk stays in flight across the barrier (the LDS contents are not
-// used).  Its time is the staging ceiling of the sweep's launch shape.
+// chunk just issued stays in flight across the barrier (a barrier without
+// the vmcnt(0) of __syncthreads; the LDS contents are not used).  The first
+// is the staging time of the sweep's double-buffered launch shape, the
+// second that of the copies alone, overlapped.
 template <int PF, int NWAVES, int INFLIGHT>
 __global__ __launch_bounds__(64 * NWAVES) void k_stage_probe(const float* __restrict__ Ju,
                                                              const uint32_t* __restrict__ pos_list,
@@ -534,11 +556,13 @@ __global__ __launch_bounds__(64 * NWAVES) void k_stage_probe(const float* __rest
         for (uint32_t k = 0; k < nchunk; k++) {
             if (wave < DMA_T / 64)
                 chunk_dma<PF, DMA_T>(rJ, voff, (pos * nchunk + k) * chb, ldsb + buf * chb, wave);
-            if constexpr (INFLIGHT == 1)
+            if constexpr (INFLIGHT == 1) {
                 wait_vmem();
-            else if (wave < DMA_T / 64)
-                __builtin_amdgcn_s_waitcnt(0x0F70 | (PF & 0xF) | ((PF >> 4) << 14));
-            __syncthreads();
+                __syncthreads();
+            } else {
+                wait_vmem_upto<PF>();
+                barrier_copies_in_flight();
+            }
             buf ^= 1;
         }
     }
