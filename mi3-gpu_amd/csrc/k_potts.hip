@@ -629,13 +629,17 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
             const uint32_t kp = i >> 5, owner = i & 31;
             const uint32_t kwp = kp >> 2, shp = (kp & 3) * 8;
             const uint32_t c0 = (i + 1) >> 5;
+            // each slot's residue at site i, from the owner lane of its
+            // half-wave (one ds_bpermute per slot; two v_readlane and a
+            // select per slot took 9-19% longer), then its row offset
+            const int own_addr = (int)((owner + (lane & 32)) * 4);
 #pragma unroll
             for (int s = 0; s < S; s++) {
-                uint32_t av = (select_word<KW>(st[s], kwp) >> shp) & 0xffu;
-                uint32_t aA = __builtin_amdgcn_readlane(av, owner);
-                uint32_t aB = __builtin_amdgcn_readlane(av, owner + 32);
-                base[s] = (upper ? aB : aA) * qch + j;
+                const uint32_t av = __builtin_amdgcn_ubfe(select_word<KW>(st[s], kwp), shp, 8);
+                base[s] = (uint32_t)__builtin_amdgcn_ds_bpermute(own_addr, (int)av);
             }
+#pragma unroll
+            for (int s = 0; s < S; s++) base[s] = mad_u24(base[s], qch, j);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 if ((uint32_t)k >= c0 && (uint32_t)k < nchunk) {
