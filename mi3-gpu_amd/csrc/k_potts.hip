@@ -613,6 +613,12 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
     float acc[S];
 #pragma unroll
     for (int s = 0; s < S; s++) acc[s] = 0.f;
+    // row offsets of the slots: in LDS bytes, so that a gather address is
+    // one SDWA shift of the partner residue and one add, for up to 16 slots
+    // per half-wave (3.05 against 4.67 ms at L=300 q=8, 7.9 against 10.2 ms
+    // at L=400 q=21); in floats for the 32 slots of two state words, where
+    // byte offsets took 3.4 against 2.5 ms at L=232 q=21
+    constexpr bool BYTE_ADDR = S <= 16;
     uint32_t base[S];
 
     const __amdgpu_buffer_rsrc_t rJ = make_rsrc(Ju, L * nchunk * chf * 4);
@@ -639,7 +645,8 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
                 base[s] = (uint32_t)__builtin_amdgcn_ds_bpermute(own_addr, (int)av);
             }
 #pragma unroll
-            for (int s = 0; s < S; s++) base[s] = mad_u24(base[s], qch, j);
+            for (int s = 0; s < S; s++)
+                base[s] = BYTE_ADDR ? mad_u24(base[s], qch * 4, j * 4) : mad_u24(base[s], qch, j);
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 if ((uint32_t)k >= c0 && (uint32_t)k < nchunk) {
@@ -653,6 +660,7 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
                                                  ldsb + (buf ^ 1) * chf * 4, wave);
                     }
                     const float* cb = lds + buf * chf;
+                    const char* cbb = ldsb + buf * chf * 4;
                     if constexpr (S > 16) {
                         // the state is loop-invariant: without this the
                         // compiler hoists the S*K extracted residues out of
@@ -671,9 +679,14 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
                         float v[G];
 #pragma unroll
                         for (int g = 0; g < G; g++) {
-                            uint32_t sm = (state_word<KW>(st[s0 + g], (uint32_t)k >> 2) >>
-                                           ((k & 3) * 8)) & 0xffu;
-                            v[g] = cb[base[s0 + g] + sm * CH];
+                            const uint32_t w = state_word<KW>(st[s0 + g], (uint32_t)k >> 2);
+                            if constexpr (BYTE_ADDR) {
+                                v[g] = *reinterpret_cast<const float*>(
+                                    cbb + base[s0 + g] + byte_x128(w, k & 3));
+                            } else {
+                                const uint32_t sm = (w >> ((k & 3) * 8)) & 0xffu;
+                                v[g] = cb[base[s0 + g] + sm * CH];
+                            }
                         }
 #pragma unroll
                         for (int g = 0; g < G; g++) acc[s0 + g] = __fmaf_rn(v[g], keep, acc[s0 + g]);
