@@ -616,8 +616,8 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
     // row offsets of the slots: in LDS bytes, so that a gather address is
     // one SDWA shift of the partner residue and one add, for up to 16 slots
     // per half-wave (3.05 against 4.67 ms at L=300 q=8, 7.9 against 10.2 ms
-    // at L=400 q=21); in floats for the 32 slots of two state words, where
-    // byte offsets took 3.4 against 2.5 ms at L=232 q=21
+    // at L=400 q=21); in floats for 32 slots, where byte offsets took 3.4
+    // against 2.5 ms at L=232 q=21 (no launch shape uses 32 slots now)
     constexpr bool BYTE_ADDR = S <= 16;
     uint32_t base[S];
 

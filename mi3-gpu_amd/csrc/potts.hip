@@ -14,12 +14,14 @@ constexpr int NT = 64 * NWAVES;
 // measured on MI355X at L=120/232/300/450, q=21 and q=4.
 // sequence slots per half-wave of the energy kernel: every staged coupling
 // chunk is read once per sequence, so more sequences per work-group cut the
-// L2 -> LDS traffic per sequence; 32 slots in 8 waves for 2 state words,
-// 16 slots in 16 waves otherwise (measured the faster)
+// L2 -> LDS traffic per sequence.  16 slots in 16 waves (512 sequences per
+// work-group) up to four state words, 8 beyond: with byte gather addresses
+// this beats 32 slots in 8 waves for two state words (2.15 against 2.45 ms
+// at L=232 q=21, 1.31 against 1.67 ms at L=200 q=8)
 template <int KW>
-constexpr int eslots() { return KW == 2 ? 32 : KW <= 4 ? 16 : 8; }
+constexpr int eslots() { return KW <= 4 ? 16 : 8; }
 template <int KW>
-constexpr int ewaves() { return KW == 2 ? 8 : 16; }
+constexpr int ewaves() { return 16; }
 
 static_assert(NT == POTTS_THREADS, "thread count");
 
