@@ -246,24 +246,42 @@ __global__ void k_weights_fixed(const float* __restrict__ w, uint32_t n,
 constexpr uint32_t WM_THREADS = 1024;
 constexpr uint32_t WM_COLS = 32;
 
+// The weights of a group: lane l holds weight l % 8 of its half-wave's 8
+// sequences (the even ones for the lower half-wave, the odd ones for the
+// upper), and the add of sequence e takes weight e from lane e of its 16-lane
+// row by DPP (row_newbcast).  Every lane loading all 8 weights of its
+// half-wave moved 64 bytes per lane through L1 for 128 distinct bytes per
+// wave, which took as long as the LDS atomics themselves.
 struct WmGroup {
-    uint4 sm;        // partner-site residues of the 16 sequences
-    uint4 si;        // row-i residues (wave-uniform)
-    uint64_t w[8];   // this half-wave's weights
+    uint4 sm;     // partner-site residues of the 16 sequences
+    uint4 si;     // row-i residues (wave-uniform)
+    uint64_t w;   // weight (lane % 8) of this half-wave's sequences
 };
 
 __device__ __forceinline__ void wm_load(WmGroup& G, const uint8_t* rowi, const uint8_t* rowm,
-                                        const uint64_t* wq, uint32_t g, bool upper) {
+                                        const uint64_t* wq, uint32_t g, uint32_t wsel) {
     G.sm = *reinterpret_cast<const uint4*>(rowm + g * 16);
     G.si = *reinterpret_cast<const uint4*>(rowi + g * 16);
-    const ulonglong2* w2 = reinterpret_cast<const ulonglong2*>(wq + (size_t)g * 16 + (upper ? 8 : 0));
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const ulonglong2 v = w2[k];
-        G.w[2 * k] = v.x;
-        G.w[2 * k + 1] = v.y;
-    }
+    G.w = wq[(size_t)g * 16 + wsel];
 }
+
+// v from lane e of each row of 16 lanes (DPP row_newbcast:e)
+#define MI3_BC(E)                                                                         \
+    case E:                                                                               \
+        lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x150 + E, 0xF, 0xF, \
+                                                   false);                                \
+        hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), 0x150 + E,  \
+                                                   0xF, 0xF, false);                      \
+        break;
+__device__ __forceinline__ uint64_t row_bcast64(uint64_t v, int e) {
+    uint32_t lo, hi;
+    switch (e) {
+        MI3_BC(0) MI3_BC(1) MI3_BC(2) MI3_BC(3) MI3_BC(4) MI3_BC(5) MI3_BC(6)
+        default: MI3_BC(7)
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+#undef MI3_BC
 
 __device__ __forceinline__ void wm_add(const WmGroup& G, unsigned long long* hist,
                                        uint32_t qcol8, uint32_t col8, uint32_t sh) {
@@ -278,7 +296,7 @@ __device__ __forceinline__ void wm_add(const WmGroup& G, unsigned long long* his
         const uint32_t b = __builtin_amdgcn_ubfe(sm[e >> 1], off, 8);
         const uint32_t addr = __umul24(a, qcol8) + b * (WM_COLS * 8) + col8;
         atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(hist) + addr),
-                  (unsigned long long)G.w[e]);
+                  (unsigned long long)row_bcast64(G.w, e));
     }
 }
 
@@ -306,25 +324,27 @@ __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __r
     const uint8_t* rowi = seq + (size_t)i * ldseq;
     const uint8_t* rowm = seq + (size_t)(valid ? m : i) * ldseq;
     const uint32_t qcol8 = q * WM_COLS * 8, col8 = col * 8, sh = upper ? 8u : 0u;
+    const uint32_t wsel = (upper ? 8u : 0u) + (lane & 7u);
     // this part's groups of 16 sequences
     const uint32_t ngroups = nseq / 16;
     const uint32_t per = (ngroups + nsplit - 1) / nsplit;
     const uint32_t g0 = part * per;
     const uint32_t g1 = (g0 + per < ngroups) ? g0 + per : ngroups;
+    // every lane adds, also those of columns with no pair (m <= i or m >= L,
+    // which read row i): the weight broadcasts need all lanes active, and
+    // the flush below skips those columns' bins
     uint32_t g = g0 + wave;
     for (; g + NW < g1; g += 2 * NW) {
         WmGroup A, B;
-        wm_load(A, rowi, rowm, wq, g, upper);
-        wm_load(B, rowi, rowm, wq, g + NW, upper);
-        if (valid) {
-            wm_add(A, hist64, qcol8, col8, sh);
-            wm_add(B, hist64, qcol8, col8, sh);
-        }
+        wm_load(A, rowi, rowm, wq, g, wsel);
+        wm_load(B, rowi, rowm, wq, g + NW, wsel);
+        wm_add(A, hist64, qcol8, col8, sh);
+        wm_add(B, hist64, qcol8, col8, sh);
     }
     if (g < g1) {
         WmGroup A;
-        wm_load(A, rowi, rowm, wq, g, upper);
-        if (valid) wm_add(A, hist64, qcol8, col8, sh);
+        wm_load(A, rowi, rowm, wq, g, wsel);
+        wm_add(A, hist64, qcol8, col8, sh);
     }
     // the last nseq % 16 sequences (weights in natural order), by the lower
     // half of the first wave of the last part
