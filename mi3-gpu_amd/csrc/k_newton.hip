@@ -333,18 +333,29 @@ __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __r
     // every lane adds, also those of columns with no pair (m <= i or m >= L,
     // which read row i): the weight broadcasts need all lanes active, and
     // the flush below skips those columns' bins
+    // (two groups per iteration, g and g + NW, the next iteration's loads
+    // issued before this one's adds.  Loads past the part's last group read
+    // that group again and are not added, so no load is under a branch: the
+    // compiler then waits for each group's own loads only.  The group bounds
+    // are wave-uniform.)
     uint32_t g = g0 + wave;
-    for (; g + NW < g1; g += 2 * NW) {
+    if (g < g1) {
+        auto clamp = [g1](uint32_t x) { return x < g1 ? x : g1 - 1; };
         WmGroup A, B;
         wm_load(A, rowi, rowm, wq, g, wsel);
-        wm_load(B, rowi, rowm, wq, g + NW, wsel);
-        wm_add(A, hist64, qcol8, col8, sh);
-        wm_add(B, hist64, qcol8, col8, sh);
-    }
-    if (g < g1) {
-        WmGroup A;
-        wm_load(A, rowi, rowm, wq, g, wsel);
-        wm_add(A, hist64, qcol8, col8, sh);
+        wm_load(B, rowi, rowm, wq, clamp(g + NW), wsel);
+        for (;;) {
+            const uint32_t gn = g + 2 * NW;
+            WmGroup An, Bn;
+            wm_load(An, rowi, rowm, wq, clamp(gn), wsel);
+            wm_load(Bn, rowi, rowm, wq, clamp(gn + NW), wsel);
+            wm_add(A, hist64, qcol8, col8, sh);
+            if (g + NW < g1) wm_add(B, hist64, qcol8, col8, sh);
+            if (gn >= g1) break;
+            A = An;
+            B = Bn;
+            g = gn;
+        }
     }
     // the last nseq % 16 sequences (weights in natural order), by the lower
     // half of the first wave of the last part
