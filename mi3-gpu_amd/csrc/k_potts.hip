@@ -208,6 +208,60 @@ __device__ __forceinline__ int half_bcast(int v, int src) {
 }
 #undef MI3_SWZ_CASE
 
+// ---------------------------------------------------------------------------
+// Chunk ring with overlapping slots, for the 57,344-byte padded chunks of
+// q = 17..21 (7 float4 per thread of 512 staging threads).  Two such chunks
+// fill 112 KB of the 160 KB of LDS, and three do not fit, so a plain ring can
+// only double-buffer: the copy of chunk n+1 starts when chunk n-1 has been
+// read and must land within the gathers of chunk n.  Here four slots
+// X0..X3 (chunk n in X[n mod 4]) overlap in pairs two apart, X0/X2 and
+// X1/X3, by 32 KB:
+//   X0 = [0, 56K)   X2 = [24K, 80K)   X1 = [80K, 136K)   X3 = [104K, 160K)
+// (K = 1024 bytes); slots one apart are disjoint.  A chunk's copy is split
+// into its part outside the partner slot (24 KB, "early") and its part
+// inside it (32 KB, "late"):
+//   * early(n+2) is issued at the start of chunk n: it overlaps none of the
+//     slots of chunks n and n+1;
+//   * late(n+1) is issued at the start of chunk n: it overlaps only the slot
+//     of chunk n-1, which every wave finished reading before the barrier
+//     that started chunk n.
+// So a chunk's copy has two chunks' time for 24 KB and one for 32 KB
+// (double buffering gives one for all 56 KB), and LDS reads never meet a
+// copy in flight.  The energy kernel streams the q = 17..21 chunks of its
+// four-state-word shape this way (for fewer state words, and in the sweep,
+// which is bound by VALU issue and LDS traffic, the ring was slower).  Per chunk each of the 8 staging waves issues 4 late then
+// 3 early one-KB copies; at the end of chunk n only early(n+2), the 3
+// youngest, may still be in flight (vmcnt 3), and the barrier that follows
+// must not wait for it (barrier_copies_in_flight).
+struct OverlapRing {
+    static constexpr uint32_t PIECE = 1024;   // one wave's copy: 64 x 16 bytes
+    static constexpr uint32_t CHB = 57344;    // padded chunk: 56 pieces
+    static constexpr int EARLY_OPS = 3, LATE_OPS = 4;  // per staging wave
+    static constexpr uint32_t BYTES = 163840;
+    // slot base (0, 80K, 24K, 104K for r = 0..3), first piece of the late
+    // part (the upper 32 pieces of X0 and X1, the lower 32 of X2 and X3) and
+    // of the early part, as arithmetic on r (scalar, no branches)
+    __device__ static uint32_t base(uint32_t r) { return (r & 1u) * 81920u + (r >> 1) * 24576u; }
+    __device__ static uint32_t late_first(uint32_t r) { return 24u - (r >> 1) * 24u; }
+    __device__ static uint32_t early_first(uint32_t r) { return (r >> 1) * 32u; }
+
+    // copy `nops` pieces starting at piece `first` of the chunk at byte
+    // offset `off` of the chunked layout into ring slot r; wave w < 8 moves
+    // pieces first + 8*o + w
+    template <int NOPS>
+    __device__ static void copy(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t off,
+                                char* lds, uint32_t r, uint32_t first, int wave) {
+        char* dst = lds + base(r) + (first + wave) * PIECE;
+        const uint32_t src = off + first * PIECE;
+#pragma unroll
+        for (int o = 0; o < NOPS; o++)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (lds_void*)(dst + o * 8 * PIECE), 16, voff,
+                                                     src + o * 8 * PIECE, 0, 0);
+    }
+};
+static_assert(OverlapRing::CHB == 7 * 512 * 16, "padded chunk of the PF = 7 class");
+static_assert(OverlapRing::BYTES == 106496 + OverlapRing::CHB, "slots fill the LDS");
+
 template <int KW>
 __device__ __forceinline__ uint32_t select_word_opaque(const uint32_t (&w)[KW], uint32_t idx) {
     if constexpr (KW == 1) return w[0];
@@ -573,14 +627,13 @@ __global__ __launch_bounds__(64 * NWAVES) void k_stage_probe(const float* __rest
 // Batch energies E(s) = sum_{i<m} J_{i,m}(s_i, s_m) of nseq sequences.
 // Restates getEnergies (mcmc.cl:249-345); same layout and staging as the
 // sweep, streaming for each row i only the chunks holding m > i.
-template <int KW, int S, int NWAVES, int PF>
+template <int KW, int S, int NWAVES, int PF, bool OVL>
 __global__ __launch_bounds__(64 * NWAVES) void k_energies(
     const float* __restrict__ Ju, const uint8_t* __restrict__ seq, float* __restrict__ E,
     uint32_t ldseq, uint32_t nseq, uint32_t L, uint32_t q, uint32_t nchunk) {
     constexpr int NT = 64 * NWAVES;
     constexpr int K = 4 * KW;
     extern __shared__ f4v lds4[];
-    float* lds = reinterpret_cast<float*>(lds4);
 
     // the padded chunk is PF float4 per thread of the first DMA_T threads
     // (the chunk layout's unit); waves beyond them take no part in staging
@@ -625,10 +678,39 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
     const uint32_t voff = tid * 16;
     char* ldsb = reinterpret_cast<char*>(lds4);
     if (L >= 2) {
-        // first unit: row 0, chunk 0
-        if (wave < DMA_T / 64) chunk_dma<PF, DMA_T>(rJ, voff, 0, ldsb, wave);
-        wait_vmem();
-        __syncthreads();
+        // OVL: copy cursor (row ci, chunk ck) two units ahead of the gathers
+        // and the byte offset of the unit one ahead; at the last unit the
+        // cursor stays, so copies past the end repeat it into unread slots
+        uint32_t ci = 0, ck = 0, off_next = 0;
+        auto advance = [&]() {
+            const bool wrap = ck + 1 >= nchunk;
+            const bool stay = wrap && ci + 2 >= L;
+            ck = stay ? ck : (wrap ? (ci + 2) >> 5 : ck + 1);
+            ci = (wrap && !stay) ? ci + 1 : ci;
+        };
+        if constexpr (OVL) {
+            static_assert(PF == 7 && DMA_T == 512, "overlapping ring: chunks of q = 17..21");
+            // unit (0, 0) whole into slot 0, then the early part of the next
+            if (wave < 8) {
+                OverlapRing::copy<OverlapRing::EARLY_OPS>(rJ, voff, 0, ldsb, 0,
+                                                          OverlapRing::early_first(0), wave);
+                OverlapRing::copy<OverlapRing::LATE_OPS>(rJ, voff, 0, ldsb, 0,
+                                                         OverlapRing::late_first(0), wave);
+            }
+            advance();
+            off_next = (ci * nchunk + ck) * chf * 4;
+            if (wave < 8)
+                OverlapRing::copy<OverlapRing::EARLY_OPS>(rJ, voff, off_next, ldsb, 1,
+                                                          OverlapRing::early_first(1), wave);
+            advance();
+            wait_vmem_upto<OverlapRing::EARLY_OPS>();
+            barrier_copies_in_flight();
+        } else {
+            // first unit: row 0, chunk 0
+            if (wave < DMA_T / 64) chunk_dma<PF, DMA_T>(rJ, voff, 0, ldsb, wave);
+            wait_vmem();
+            __syncthreads();
+        }
         uint32_t buf = 0;
         const uint32_t qch = q * CH;
         for (uint32_t i = 0; i + 1 < L; i++) {
@@ -650,17 +732,33 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 if ((uint32_t)k >= c0 && (uint32_t)k < nchunk) {
-                    const bool last_chunk = (uint32_t)k + 1 == nchunk;
-                    const bool has_next = !last_chunk || (i + 2 < L);
-                    if (has_next) {
-                        uint32_t ni = last_chunk ? i + 1 : i;
-                        uint32_t nc = last_chunk ? ((i + 2) >> 5) : (uint32_t)k + 1;
-                        if (wave < DMA_T / 64)
-                            chunk_dma<PF, DMA_T>(rJ, voff, (ni * nchunk + nc) * chf * 4,
-                                                 ldsb + (buf ^ 1) * chf * 4, wave);
+                    const char* cbb;
+                    if constexpr (OVL) {
+                        // late part of the next unit, early part of the one after
+                        if (wave < 8) {
+                            OverlapRing::copy<OverlapRing::LATE_OPS>(
+                                rJ, voff, off_next, ldsb, (buf + 1) & 3,
+                                OverlapRing::late_first((buf + 1) & 3), wave);
+                            OverlapRing::copy<OverlapRing::EARLY_OPS>(
+                                rJ, voff, (ci * nchunk + ck) * chf * 4, ldsb, (buf + 2) & 3,
+                                OverlapRing::early_first((buf + 2) & 3), wave);
+                        }
+                        off_next = (ci * nchunk + ck) * chf * 4;
+                        advance();
+                        cbb = ldsb + OverlapRing::base(buf);
+                    } else {
+                        const bool last_chunk = (uint32_t)k + 1 == nchunk;
+                        const bool has_next = !last_chunk || (i + 2 < L);
+                        if (has_next) {
+                            uint32_t ni = last_chunk ? i + 1 : i;
+                            uint32_t nc = last_chunk ? ((i + 2) >> 5) : (uint32_t)k + 1;
+                            if (wave < DMA_T / 64)
+                                chunk_dma<PF, DMA_T>(rJ, voff, (ni * nchunk + nc) * chf * 4,
+                                                     ldsb + (buf ^ 1) * chf * 4, wave);
+                        }
+                        cbb = ldsb + buf * chf * 4;
                     }
-                    const float* cb = lds + buf * chf;
-                    const char* cbb = ldsb + buf * chf * 4;
+                    const float* cb = reinterpret_cast<const float*>(cbb);
                     if constexpr (S > 16) {
                         // the state is loop-invariant: without this the
                         // compiler hoists the S*K extracted residues out of
@@ -691,13 +789,23 @@ __global__ __launch_bounds__(64 * NWAVES) void k_energies(
 #pragma unroll
                         for (int g = 0; g < G; g++) acc[s0 + g] = __fmaf_rn(v[g], keep, acc[s0 + g]);
                     }
-                    wait_vmem();
-                    __syncthreads();
-                    buf ^= 1;
+                    if constexpr (OVL) {
+                        // the next unit has landed (only the early part of
+                        // the one after may be in flight)
+                        wait_vmem_upto<OverlapRing::EARLY_OPS>();
+                        barrier_copies_in_flight();
+                        buf = (buf + 1) & 3;
+                    } else {
+                        wait_vmem();
+                        __syncthreads();
+                        buf ^= 1;
+                    }
                 }
             }
         }
     }
+    // copies issued past the last unit
+    if constexpr (OVL) wait_vmem();
 #pragma unroll
     for (int s = 0; s < S; s++) {
         float e = half_wave_sum(acc[s]);

@@ -50,8 +50,14 @@ template <int KW, int PF>
 hipError_t energies_kw(const EnergyArgs& a, hipStream_t st) {
     constexpr int NW = ewaves<KW>();
     constexpr uint32_t W = 2 * eslots<KW>() * NW;
-    auto kern = k_energies<KW, eslots<KW>(), NW, PF>;
-    size_t lds = 2ull * chunk_floats(a.q) * sizeof(float);
+    // with four state words (L = 385..512), the 57 KB chunks of q = 17..21
+    // stream through the overlapping four-slot ring (all 160 KB of LDS):
+    // 7.10 against 7.94 ms at L=400, 10.7 against 11.8 ms at L=500.  The
+    // others are double-buffered (the ring took 6% longer at L=300 and 1% at
+    // L=232)
+    constexpr bool OVL = PF == 7 && KW == 4;
+    auto kern = k_energies<KW, eslots<KW>(), NW, PF, OVL>;
+    size_t lds = OVL ? (size_t)OverlapRing::BYTES : 2ull * chunk_floats(a.q) * sizeof(float);
     hipError_t e = set_lds(kern, lds);
     if (e != hipSuccess) return e;
     dim3 grid((a.nseq + W - 1) / W);

@@ -27,7 +27,7 @@ def make_gpu(tmp_path, L, q, N, seed=1234, nsteps=None, profile=False, wgsize=No
 
 
 SHAPES = [(99, 4), (232, 21), (37, 7), (5, 3), (300, 21), (64, 21), (33, 2), (450, 4),
-          (700, 21), (1000, 4)]
+          (700, 21), (1000, 4), (400, 21), (500, 17)]
 
 
 @pytest.mark.parametrize("L,q", SHAPES)
@@ -44,6 +44,24 @@ def test_energies_match_oracle(gpu_ok, tmp_path, L, q):
     scale = cboracle.energies(s, np.abs(J))
     err = np.abs(E - Ex) / scale
     print(f"L={L} q={q} max rel err (vs sum|terms|) {err.max():.3g}")
+    assert err.max() < 1e-5
+
+
+def test_energies_four_words_ring_many_groups(gpu_ok, tmp_path):
+    """Four state words at q=21 (the energy kernel's overlapping chunk ring)
+    over five work-groups."""
+    L, q, N = 420, 21, 2560
+    J = model(L, q, 4)
+    s = seqs_rand(N, L, q, 5)
+    g = make_gpu(tmp_path, L, q, N)
+    g.setBuf('J', J)
+    g.setBuf('seq main', s)
+    g.calcEnergies('main')
+    E = g.getBuf('E main').read()
+    Ex = cboracle.energies(s, J)
+    scale = cboracle.energies(s, np.abs(J))
+    err = np.abs(E - Ex) / scale
+    print(f"L={L} q={q} N={N} max rel err (vs sum|terms|) {err.max():.3g}")
     assert err.max() < 1e-5
 
 
