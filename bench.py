@@ -375,6 +375,19 @@ def measure_newton_kernels(n=2 ** 18, L=232, q=21, reps=5, seed=9):
                   flush=True)
         en_ms, _ = g.kernel_ms('getEnergies')
         wm_ms, _ = g.kernel_ms('weightedMarg')
+        # the weighted marginals' 64-bit LDS atomic adds alone (same
+        # work-group size, histogram and column pattern, no loads)
+        from mi3gpu._lib import lib, check
+        ap_wgs, ap_iters = 8 * 256, 2048
+        for _ in range(4):
+            check(g._ctx, lib().mi3_wm_atomic_probe(g._ctx, ap_wgs, ap_iters))
+        g.wait()
+        g.profile_reset()
+        for _ in range(3):
+            check(g._ctx, lib().mi3_wm_atomic_probe(g._ctx, ap_wgs, ap_iters))
+        g.wait()
+        ap_ms, _ = g.kernel_ms('wmAtomicProbe')
+        ceiling = ap_wgs * 1024.0 * ap_iters / (ap_ms * 1e-3)
         del g
     units = sum(max(0, (L + 31) // 32 - (i + 1) // 32) for i in range(L - 1))
     chunk_bytes = _pf_class(q) * 512 * 16
@@ -384,7 +397,9 @@ def measure_newton_kernels(n=2 ** 18, L=232, q=21, reps=5, seed=9):
             "energies_staging_bytes": stage,
             "energies_staging_TBps": stage / (en_ms * 1e-3) / 1e12,
             "weighted_marg_ms": wm_ms,
-            "weighted_marg_u64_adds_per_s": float(n) * nP / (wm_ms * 1e-3)}
+            "weighted_marg_u64_adds_per_s": float(n) * nP / (wm_ms * 1e-3),
+            "weighted_marg_atomic_ceiling_adds_per_s": ceiling,
+            "weighted_marg_frac_of_atomic_ceiling": float(n) * nP / (wm_ms * 1e-3) / ceiling}
 
 
 def _best_wgsize(J, s0, L, q, seed):

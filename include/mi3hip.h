@@ -172,6 +172,14 @@ int mi3_run_mcmc(mi3_ctx *ctx, float beta);
  * as "stageProbe" / "stageProbe2"; the walker state is not touched. */
 int mi3_stage_probe(mi3_ctx *ctx, int inflight);
 
+/* Measurement probe with no counterpart in the reference: the 64-bit LDS
+ * atomic adds of mi3_weighted_marg alone, in its work-group size (1024
+ * threads), histogram (q*q rows of 32 uint64) and column pattern, to
+ * pseudo-random rows, with no global loads.  work_groups (1..65536) groups
+ * of 1024 threads make iters (1..2^20) adds each.  Recorded in the profile
+ * as "wmAtomicProbe". */
+int mi3_wm_atomic_probe(mi3_ctx *ctx, uint32_t work_groups, uint32_t iters);
+
 /* Number of MCMC steps executed so far on this context (the global step
  * counter of the random streams). */
 int mi3_mcmc_step_count(mi3_ctx *ctx, uint64_t *steps);

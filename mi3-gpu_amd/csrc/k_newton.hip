@@ -374,6 +374,30 @@ __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __r
     }
 }
 
+// Measurement probe: the 64-bit LDS atomic rate of k_weighted_marg's add
+// pattern alone.  The same work-group size and histogram (q*q rows of 32
+// uint64 in LDS); lane l adds into column l % 32 of a pseudo-random row (a
+// 32-bit LCG per thread), so the two half-waves of a wave share columns as
+// in the kernel; no global loads.  Each thread makes `iters` adds; the
+// histogram's sum goes to out[0] so that nothing is dropped.
+__global__ __launch_bounds__(WM_THREADS) void k_wm_atomic_probe(uint32_t q, uint32_t iters,
+                                                                unsigned long long* out) {
+    extern __shared__ unsigned long long hist64[];
+    const uint32_t qq = q * q, tid = threadIdx.x, col = tid & (WM_COLS - 1);
+    for (uint32_t e = tid; e < qq * WM_COLS; e += WM_THREADS) hist64[e] = 0ull;
+    __syncthreads();
+    uint32_t r = (blockIdx.x * WM_THREADS + tid) * 2654435761u + 12345u;
+    for (uint32_t it = 0; it < iters; it++) {
+        r = r * 1664525u + 1013904223u;
+        const uint32_t row = __umulhi(r, qq);  // in [0, q*q)
+        atomicAdd(&hist64[row * WM_COLS + col], (unsigned long long)(r | 1u));
+    }
+    __syncthreads();
+    unsigned long long sum = 0;
+    for (uint32_t e = tid; e < qq * WM_COLS; e += WM_THREADS) sum += hist64[e];
+    atomicAdd(out, sum);
+}
+
 // Pass 3: bins of 2^-62 sum(w) to float (NaN if the total was bad).
 __global__ void k_fixed_to_float(const unsigned long long* __restrict__ acc,
                                  float* __restrict__ out, uint64_t n,

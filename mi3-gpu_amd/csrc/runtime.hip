@@ -95,6 +95,7 @@ struct mi3_ctx {
     uint64_t* wq = nullptr;                   // fixed-point weights
     size_t wq_cap = 0;
     unsigned long long* wacc = nullptr;       // fixed-point weighted-marginal bins
+    unsigned long long* probe_out = nullptr;  // sink of mi3_wm_atomic_probe
     uint32_t* wbad = nullptr;
     uint32_t* wm_units = nullptr;
     uint32_t n_units = 0;
@@ -405,6 +406,7 @@ int mi3_destroy(mi3_ctx* c) {
     hipFree(c->red_out);
     hipFree(c->wq);
     hipFree(c->wacc);
+    hipFree(c->probe_out);
     hipFree(c->wbad);
     hipFree(c->wm_units);
     hipFree(c->seq1);
@@ -702,6 +704,21 @@ int mi3_run_mcmc(mi3_ctx* c, float beta) {
     }
     c->step += c->nsteps + (c->nsteps & 1u);  // keep the global step even
     sb->version++;
+    return MI3_OK;
+}
+
+int mi3_wm_atomic_probe(mi3_ctx* c, uint32_t work_groups, uint32_t iters) {
+    if (!c) return MI3_EINVAL;
+    if (work_groups == 0 || work_groups > 65536 || iters == 0 || iters > (1u << 20))
+        return fail(c, MI3_EINVAL, "work_groups must be 1..65536 and iters 1..2^20");
+    if (!c->probe_out) HIPCHK(c, hipMalloc(&c->probe_out, sizeof(unsigned long long)));
+    const size_t lds = (size_t)c->q * c->q * WM_COLS * sizeof(unsigned long long);
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(k_wm_atomic_probe),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    Scope s(c, "wmAtomicProbe", (size_t)work_groups * WM_THREADS * iters * 8);
+    hipLaunchKernelGGL(k_wm_atomic_probe, dim3(work_groups), dim3(WM_THREADS), lds, c->stream,
+                       c->q, iters, c->probe_out);
+    HIPCHK(c, hipGetLastError());
     return MI3_OK;
 }
 

@@ -46,6 +46,7 @@ SIGNATURES = {
     'mi3_init_mcmc': (_i, [_P, _u32, _u64, _u64]),
     'mi3_run_mcmc': (_i, [_P, _f]),
     'mi3_stage_probe': (_i, [_P, _i]),
+    'mi3_wm_atomic_probe': (_i, [_P, _u32, _u32]),
     'mi3_mcmc_step_count': (_i, [_P, _c.POINTER(_u64)]),
     'mi3_gen_indep': (_i, [_P, _S]),
     'mi3_calc_energies': (_i, [_P, _S, _S, _S, _u32]),
