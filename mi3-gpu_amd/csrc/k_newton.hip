@@ -265,13 +265,13 @@ __device__ __forceinline__ void wm_load(WmGroup& G, const uint8_t* rowi, const u
     G.w = wq[(size_t)g * 16 + wsel];
 }
 
-// v from lane e of each row of 16 lanes (DPP row_newbcast:e)
-#define MI3_BC(E)                                                                         \
-    case E:                                                                               \
-        lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x150 + E, 0xF, 0xF, \
-                                                   false);                                \
-        hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), 0x150 + E,  \
-                                                   0xF, 0xF, false);                      \
+// v from lane e of each row of 16 lanes (DPP row_newbcast:e; every source
+// lane is active, so the destination needs no initial value)
+#define MI3_BC(E)                                                                       \
+    case E:                                                                             \
+        lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)v, 0x150 + E, 0xF, 0xF, false); \
+        hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(v >> 32), 0x150 + E, 0xF, 0xF, \
+                                                false);                                 \
         break;
 __device__ __forceinline__ uint64_t row_bcast64(uint64_t v, int e) {
     uint32_t lo, hi;
@@ -283,19 +283,45 @@ __device__ __forceinline__ uint64_t row_bcast64(uint64_t v, int e) {
 }
 #undef MI3_BC
 
-__device__ __forceinline__ void wm_add(const WmGroup& G, unsigned long long* hist,
-                                       uint32_t qcol8, uint32_t col8, uint32_t sh) {
-    // the upper half-wave takes the odd sequence of each pair: byte offsets
-    // of its residues are one higher
-    const uint32_t si[4] = {G.si.x, G.si.y, G.si.z, G.si.w};
-    const uint32_t sm[4] = {G.sm.x, G.sm.y, G.sm.z, G.sm.w};
+// n * byte B of w (B = 0 or 2) in one VALU op: an SDWA byte select on the
+// multiplier, and 256 * byte B of w as a shift with the same select
+template <int B>
+__device__ __forceinline__ uint32_t mul_byte(uint32_t n, uint32_t w) {
+    uint32_t r;
+    if constexpr (B == 0)
+        asm("v_mul_u32_u24_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD "
+            "src0_sel:DWORD src1_sel:BYTE_0" : "=v"(r) : "v"(n), "v"(w));
+    else
+        asm("v_mul_u32_u24_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD "
+            "src0_sel:DWORD src1_sel:BYTE_2" : "=v"(r) : "v"(n), "v"(w));
+    return r;
+}
+template <int B>
+__device__ __forceinline__ uint32_t byte_x256(uint32_t w) {
+    uint32_t r;
+    if constexpr (B == 0)
+        asm("v_lshlrev_b32_sdwa %0, 8, %1 dst_sel:DWORD dst_unused:UNUSED_PAD "
+            "src0_sel:DWORD src1_sel:BYTE_0" : "=v"(r) : "v"(w));
+    else
+        asm("v_lshlrev_b32_sdwa %0, 8, %1 dst_sel:DWORD dst_unused:UNUSED_PAD "
+            "src0_sel:DWORD src1_sel:BYTE_2" : "=v"(r) : "v"(w));
+    return r;
+}
+
+// hcol: the histogram's base plus this lane's column offset (formed once, so
+// that an address is one add of the row and residue offsets to it)
+__device__ __forceinline__ void wm_add(const WmGroup& G, char* hcol, uint32_t qcol8,
+                                       uint32_t sh) {
+    // the upper half-wave takes the odd sequence of each pair, whose residues
+    // are one byte higher: shifted down once per group, every lane reads
+    // bytes 0 and 2 of each word (sequences 4k + 2(e & 1) + upper)
+    const uint32_t si[4] = {G.si.x >> sh, G.si.y >> sh, G.si.z >> sh, G.si.w >> sh};
+    const uint32_t sm[4] = {G.sm.x >> sh, G.sm.y >> sh, G.sm.z >> sh, G.sm.w >> sh};
 #pragma unroll
     for (int e = 0; e < 8; e++) {
-        const uint32_t off = sh + 16u * (e & 1);
-        const uint32_t a = __builtin_amdgcn_ubfe(si[e >> 1], off, 8);
-        const uint32_t b = __builtin_amdgcn_ubfe(sm[e >> 1], off, 8);
-        const uint32_t addr = __umul24(a, qcol8) + b * (WM_COLS * 8) + col8;
-        atomicAdd(reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(hist) + addr),
+        const uint32_t arow = (e & 1) ? mul_byte<2>(qcol8, si[e >> 1]) : mul_byte<0>(qcol8, si[e >> 1]);
+        const uint32_t bcol = (e & 1) ? byte_x256<2>(sm[e >> 1]) : byte_x256<0>(sm[e >> 1]);
+        atomicAdd(reinterpret_cast<unsigned long long*>(hcol + (arow + bcol)),
                   (unsigned long long)row_bcast64(G.w, e));
     }
 }
@@ -325,6 +351,7 @@ __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __r
     const uint8_t* rowm = seq + (size_t)(valid ? m : i) * ldseq;
     const uint32_t qcol8 = q * WM_COLS * 8, col8 = col * 8, sh = upper ? 8u : 0u;
     const uint32_t wsel = (upper ? 8u : 0u) + (lane & 7u);
+    char* hcol = reinterpret_cast<char*>(hist64) + col8;
     // this part's groups of 16 sequences
     const uint32_t ngroups = nseq / 16;
     const uint32_t per = (ngroups + nsplit - 1) / nsplit;
@@ -349,8 +376,8 @@ __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __r
             WmGroup An, Bn;
             wm_load(An, rowi, rowm, wq, clamp(gn), wsel);
             wm_load(Bn, rowi, rowm, wq, clamp(gn + NW), wsel);
-            wm_add(A, hist64, qcol8, col8, sh);
-            if (g + NW < g1) wm_add(B, hist64, qcol8, col8, sh);
+            wm_add(A, hcol, qcol8, sh);
+            if (g + NW < g1) wm_add(B, hcol, qcol8, sh);
             if (gn >= g1) break;
             A = An;
             B = Bn;
