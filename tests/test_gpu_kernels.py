@@ -65,6 +65,21 @@ def test_energies_four_words_ring_many_groups(gpu_ok, tmp_path):
     assert err.max() < 1e-5
 
 
+def test_wm_atomic_probe_runs_and_checks_sizes(gpu_ok, tmp_path):
+    """The weighted marginals' atomic-ceiling probe runs in bounds (q=21, the
+    largest histogram) and refuses sizes outside its documented limits."""
+    from mi3gpu._lib import lib
+    g = make_gpu(tmp_path, 40, 21, 512, profile=True)
+    assert lib().mi3_wm_atomic_probe(g._ctx, 8, 16) == 0
+    g.wait()
+    ms, n = g.kernel_ms('wmAtomicProbe')
+    assert n == 1 and ms > 0
+    assert lib().mi3_wm_atomic_probe(g._ctx, 0, 16) != 0
+    assert lib().mi3_wm_atomic_probe(g._ctx, 8, 0) != 0
+    assert lib().mi3_wm_atomic_probe(g._ctx, 65537, 16) != 0
+    assert lib().mi3_wm_atomic_probe(g._ctx, 8, (1 << 20) + 1) != 0
+
+
 def test_energies_special_cases(gpu_ok, tmp_path):
     L, q, N = 40, 5, 512
     g = make_gpu(tmp_path, L, q, N)
