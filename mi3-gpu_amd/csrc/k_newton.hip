@@ -1,6 +1,8 @@
 // Quasi-Newton (Zwanzig reweighting) kernels of one inference round.
 #include "common.h"
 
+#include <utility>
+
 namespace mi3 {
 
 // bimarg = float(count) / float(nseq), correctly rounded (mcmc.cl:518-528).
@@ -214,11 +216,8 @@ __global__ void k_add_floats(float* __restrict__ dst, const float* __restrict__ 
 // and the result does not depend on the order of any addition.
 //
 // Pass 1: fixed-point weights (a non-finite or zero total makes every output
-// NaN, which keeps the divergence check of the Newton loop working).  Within
-// each whole group of 16 sequences the even-numbered weights are stored
-// first and the odd-numbered ones after them, so that each half-wave of the
-// histogram kernel reads its 8 weights as contiguous vector loads; the tail
-// past the last whole group keeps its natural order.
+// NaN, which keeps the divergence check of the Newton loop working), in the
+// sequences' order.
 __global__ void k_weights_fixed(const float* __restrict__ w, uint32_t n,
                                 const float* __restrict__ wsum, uint64_t* __restrict__ wq,
                                 uint32_t* __restrict__ bad) {
@@ -228,41 +227,44 @@ __global__ void k_weights_fixed(const float* __restrict__ w, uint32_t n,
     if (i == 0) bad[0] = ok ? 0u : 1u;
     if (i >= n) return;
     const double v = ok ? (double)w[i] * (4611686018427387904.0 / tot) : 0.0;  // 2^62 / sum
-    const uint32_t r = i & 15u;
-    const uint32_t dst = (i < (n & ~15u)) ? (i & ~15u) + ((r & 1u) ? 8u : 0u) + (r >> 1) : i;
-    wq[dst] = (v > 0.0) ? (uint64_t)(v + 0.5) : 0ull;
+    wq[i] = (v > 0.0) ? (uint64_t)(v + 0.5) : 0ull;
 }
 
-// Pass 2: one work-group per (row i, group of 32 partner sites m > i, part
-// of the sequence range).  The 32 pair histograms live in LDS as uint64
-// [a*q+b][32]: lane l of a wave owns pair (i, 32*cp + (l & 31)), and the two
-// half-waves add two consecutive sequences (the lower half the even one),
-// so the row-i residues are wave-uniform (scalar) loads.  The 16 waves take
-// groups of 16 sequences in turn, two groups per iteration with all their
-// loads issued before the first add.  The part's histograms are added into
-// the uint64 accumulator (zeroed beforehand) with integer atomics; the
-// accumulator holds the pairs from pair_lo on (units are ordered by row,
-// then partner group, so a run of units covers a contiguous pair range).
+// Pass 2: one work-group per (unit of 32 consecutive pairs, part of the
+// sequence range).  Unit u holds pairs 32u .. 32u + 31 of the packed order,
+// rows chained (a unit can span rows), so only the last unit has columns
+// with no pair.  The 32 pair histograms live in LDS as uint64 [a*q+b][32]:
+// lane l of a wave owns pair 32u + (l & 31).  A wave takes groups of 32
+// sequences, the lower half-wave the first 16 and the upper half-wave the
+// next 16, so that one 16-byte load per lane of each half-wave reads both
+// halves of a 32-byte piece of the same row: each cache line looked up
+// serves 16 adds per lane.  (With two half-waves adding the even and odd
+// sequences of one group of 16, each lookup served 8, and the vector memory
+// path, not the LDS atomics, set the pace.)  The 16 waves take groups in
+// turn, two groups per iteration with all their loads issued before the
+// first add.  The part's histograms are added into the uint64 accumulator
+// (zeroed beforehand) with integer atomics; the accumulator holds the pairs
+// from pair_lo on (a run of units covers a contiguous pair range).
 constexpr uint32_t WM_THREADS = 1024;
 constexpr uint32_t WM_COLS = 32;
 
-// The weights of a group: lane l holds weight l % 8 of its half-wave's 8
-// sequences (the even ones for the lower half-wave, the odd ones for the
-// upper), and the add of sequence e takes weight e from lane e of its 16-lane
-// row by DPP (row_newbcast).  Every lane loading all 8 weights of its
-// half-wave moved 64 bytes per lane through L1 for 128 distinct bytes per
-// wave, which took as long as the LDS atomics themselves.
+// One group for one lane: the residues of its half-wave's 16 sequences at
+// the pair's partner site and row site, and weight (lane % 16) of those
+// sequences; the add of sequence e takes weight e from lane e of its 16-lane
+// row by DPP (row_newbcast), so that a lane loads one weight, not 16.
 struct WmGroup {
-    uint4 sm;     // partner-site residues of the 16 sequences
-    uint4 si;     // row-i residues (wave-uniform)
-    uint64_t w;   // weight (lane % 8) of this half-wave's sequences
+    uint4 sm;     // partner-site residues of the half-wave's 16 sequences
+    uint4 si;     // row-site residues of the same sequences
+    uint64_t w;   // weight (lane % 16) of those sequences
 };
 
+// half: 16 for the upper half-wave, else 0; wsel = half + lane % 16
 __device__ __forceinline__ void wm_load(WmGroup& G, const uint8_t* rowi, const uint8_t* rowm,
-                                        const uint64_t* wq, uint32_t g, uint32_t wsel) {
-    G.sm = *reinterpret_cast<const uint4*>(rowm + g * 16);
-    G.si = *reinterpret_cast<const uint4*>(rowi + g * 16);
-    G.w = wq[(size_t)g * 16 + wsel];
+                                        const uint64_t* wq, uint32_t g, uint32_t half,
+                                        uint32_t wsel) {
+    G.sm = *reinterpret_cast<const uint4*>(rowm + g * 32 + half);
+    G.si = *reinterpret_cast<const uint4*>(rowi + g * 32 + half);
+    G.w = wq[(size_t)g * 32 + wsel];
 }
 
 // v from lane e of each row of 16 lanes (DPP row_newbcast:e; every source
@@ -276,54 +278,59 @@ __device__ __forceinline__ void wm_load(WmGroup& G, const uint8_t* rowi, const u
 __device__ __forceinline__ uint64_t row_bcast64(uint64_t v, int e) {
     uint32_t lo, hi;
     switch (e) {
-        MI3_BC(0) MI3_BC(1) MI3_BC(2) MI3_BC(3) MI3_BC(4) MI3_BC(5) MI3_BC(6)
-        default: MI3_BC(7)
+        MI3_BC(0) MI3_BC(1) MI3_BC(2) MI3_BC(3) MI3_BC(4) MI3_BC(5) MI3_BC(6) MI3_BC(7)
+        MI3_BC(8) MI3_BC(9) MI3_BC(10) MI3_BC(11) MI3_BC(12) MI3_BC(13) MI3_BC(14)
+        default: MI3_BC(15)
     }
     return ((uint64_t)hi << 32) | lo;
 }
 #undef MI3_BC
 
-// n * byte B of w (B = 0 or 2) in one VALU op: an SDWA byte select on the
-// multiplier, and 256 * byte B of w as a shift with the same select
+// n * byte B of w in one VALU op (an SDWA byte select on the multiplier),
+// and 256 * byte B of w as a shift with the same select
+#define MI3_SDWA_BYTE(B)                                                                     \
+    template <>                                                                              \
+    __device__ __forceinline__ uint32_t mul_byte<B>(uint32_t n, uint32_t w) {                \
+        uint32_t r;                                                                          \
+        asm("v_mul_u32_u24_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD "             \
+            "src0_sel:DWORD src1_sel:BYTE_" #B : "=v"(r) : "v"(n), "v"(w));                  \
+        return r;                                                                            \
+    }                                                                                        \
+    template <>                                                                              \
+    __device__ __forceinline__ uint32_t byte_x256<B>(uint32_t w) {                           \
+        uint32_t r;                                                                          \
+        asm("v_lshlrev_b32_sdwa %0, 8, %1 dst_sel:DWORD dst_unused:UNUSED_PAD "              \
+            "src0_sel:DWORD src1_sel:BYTE_" #B : "=v"(r) : "v"(w));                          \
+        return r;                                                                            \
+    }
 template <int B>
-__device__ __forceinline__ uint32_t mul_byte(uint32_t n, uint32_t w) {
-    uint32_t r;
-    if constexpr (B == 0)
-        asm("v_mul_u32_u24_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD "
-            "src0_sel:DWORD src1_sel:BYTE_0" : "=v"(r) : "v"(n), "v"(w));
-    else
-        asm("v_mul_u32_u24_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD "
-            "src0_sel:DWORD src1_sel:BYTE_2" : "=v"(r) : "v"(n), "v"(w));
-    return r;
-}
+__device__ __forceinline__ uint32_t mul_byte(uint32_t n, uint32_t w);
 template <int B>
-__device__ __forceinline__ uint32_t byte_x256(uint32_t w) {
-    uint32_t r;
-    if constexpr (B == 0)
-        asm("v_lshlrev_b32_sdwa %0, 8, %1 dst_sel:DWORD dst_unused:UNUSED_PAD "
-            "src0_sel:DWORD src1_sel:BYTE_0" : "=v"(r) : "v"(w));
-    else
-        asm("v_lshlrev_b32_sdwa %0, 8, %1 dst_sel:DWORD dst_unused:UNUSED_PAD "
-            "src0_sel:DWORD src1_sel:BYTE_2" : "=v"(r) : "v"(w));
-    return r;
-}
+__device__ __forceinline__ uint32_t byte_x256(uint32_t w);
+MI3_SDWA_BYTE(0)
+MI3_SDWA_BYTE(1)
+MI3_SDWA_BYTE(2)
+MI3_SDWA_BYTE(3)
+#undef MI3_SDWA_BYTE
 
 // hcol: the histogram's base plus this lane's column offset (formed once, so
 // that an address is one add of the row and residue offsets to it)
-__device__ __forceinline__ void wm_add(const WmGroup& G, char* hcol, uint32_t qcol8,
-                                       uint32_t sh) {
-    // the upper half-wave takes the odd sequence of each pair, whose residues
-    // are one byte higher: shifted down once per group, every lane reads
-    // bytes 0 and 2 of each word (sequences 4k + 2(e & 1) + upper)
-    const uint32_t si[4] = {G.si.x >> sh, G.si.y >> sh, G.si.z >> sh, G.si.w >> sh};
-    const uint32_t sm[4] = {G.sm.x >> sh, G.sm.y >> sh, G.sm.z >> sh, G.sm.w >> sh};
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const uint32_t arow = (e & 1) ? mul_byte<2>(qcol8, si[e >> 1]) : mul_byte<0>(qcol8, si[e >> 1]);
-        const uint32_t bcol = (e & 1) ? byte_x256<2>(sm[e >> 1]) : byte_x256<0>(sm[e >> 1]);
-        atomicAdd(reinterpret_cast<unsigned long long*>(hcol + (arow + bcol)),
-                  (unsigned long long)row_bcast64(G.w, e));
-    }
+template <int E>
+__device__ __forceinline__ void wm_add1(const WmGroup& G, char* hcol, uint32_t qcol8) {
+    const uint32_t si = (E >> 2) == 0 ? G.si.x : (E >> 2) == 1 ? G.si.y : (E >> 2) == 2 ? G.si.z : G.si.w;
+    const uint32_t sm = (E >> 2) == 0 ? G.sm.x : (E >> 2) == 1 ? G.sm.y : (E >> 2) == 2 ? G.sm.z : G.sm.w;
+    const uint32_t arow = mul_byte<E & 3>(qcol8, si);
+    const uint32_t bcol = byte_x256<E & 3>(sm);
+    atomicAdd(reinterpret_cast<unsigned long long*>(hcol + (arow + bcol)),
+              (unsigned long long)row_bcast64(G.w, E));
+}
+template <int... E>
+__device__ __forceinline__ void wm_add_seq(const WmGroup& G, char* hcol, uint32_t qcol8,
+                                           std::integer_sequence<int, E...>) {
+    (wm_add1<E>(G, hcol, qcol8), ...);
+}
+__device__ __forceinline__ void wm_add(const WmGroup& G, char* hcol, uint32_t qcol8) {
+    wm_add_seq(G, hcol, qcol8, std::make_integer_sequence<int, 16>{});
 }
 
 __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __restrict__ seq,
@@ -337,29 +344,40 @@ __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __r
     extern __shared__ unsigned long long hist64[];
     constexpr uint32_t NW = WM_THREADS / 64;
     const uint32_t unit = units[blockIdx.x / nsplit], part = blockIdx.x % nsplit;
-    const uint32_t i = unit >> 8, cp = unit & 0xff;
     const uint32_t qq = q * q;
+    const uint32_t npairs = L * (L - 1) / 2;
     const uint32_t tid = threadIdx.x;
     for (uint32_t e = tid; e < qq * WM_COLS; e += WM_THREADS) hist64[e] = 0ull;
     __syncthreads();
     const uint32_t lane = tid & 63, col = lane & (WM_COLS - 1);
     const bool upper = lane >= 32;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
-    const uint32_t m = cp * WM_COLS + col;
-    const bool valid = (m > i) && (m < L);
+    const uint32_t p0 = unit * WM_COLS;
+    const bool valid = p0 + col < npairs;
+    // sites (i, m) of the lane's pair: i is the last row whose first pair
+    // index i(2L - i - 1)/2 is <= p.  Columns past the last pair count the
+    // last pair again and are not flushed.
+    const uint32_t p = valid ? p0 + col : npairs - 1;
+    uint32_t lo = 0, hi = L - 1;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (mid * (2 * L - mid - 1) / 2 <= p) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t i = lo, m = p - lo * (2 * L - lo - 1) / 2 + lo + 1;
     const uint8_t* rowi = seq + (size_t)i * ldseq;
-    const uint8_t* rowm = seq + (size_t)(valid ? m : i) * ldseq;
-    const uint32_t qcol8 = q * WM_COLS * 8, col8 = col * 8, sh = upper ? 8u : 0u;
-    const uint32_t wsel = (upper ? 8u : 0u) + (lane & 7u);
+    const uint8_t* rowm = seq + (size_t)m * ldseq;
+    const uint32_t qcol8 = q * WM_COLS * 8, col8 = col * 8;
+    const uint32_t half = upper ? 16u : 0u, wsel = half + (lane & 15u);
     char* hcol = reinterpret_cast<char*>(hist64) + col8;
-    // this part's groups of 16 sequences
-    const uint32_t ngroups = nseq / 16;
+    // this part's groups of 32 sequences
+    const uint32_t ngroups = nseq / 32;
     const uint32_t per = (ngroups + nsplit - 1) / nsplit;
     const uint32_t g0 = part * per;
     const uint32_t g1 = (g0 + per < ngroups) ? g0 + per : ngroups;
-    // every lane adds, also those of columns with no pair (m <= i or m >= L,
-    // which read row i): the weight broadcasts need all lanes active, and
-    // the flush below skips those columns' bins
+    // every lane adds, also those of columns past the last pair: the weight
+    // broadcasts need all lanes active, and the flush below skips those
+    // columns' bins
     // (two groups per iteration, g and g + NW, the next iteration's loads
     // issued before this one's adds.  Loads past the part's last group read
     // that group again and are not added, so no load is under a branch: the
@@ -369,35 +387,33 @@ __global__ __launch_bounds__(WM_THREADS) void k_weighted_marg(const uint8_t* __r
     if (g < g1) {
         auto clamp = [g1](uint32_t x) { return x < g1 ? x : g1 - 1; };
         WmGroup A, B;
-        wm_load(A, rowi, rowm, wq, g, wsel);
-        wm_load(B, rowi, rowm, wq, clamp(g + NW), wsel);
+        wm_load(A, rowi, rowm, wq, g, half, wsel);
+        wm_load(B, rowi, rowm, wq, clamp(g + NW), half, wsel);
         for (;;) {
             const uint32_t gn = g + 2 * NW;
             WmGroup An, Bn;
-            wm_load(An, rowi, rowm, wq, clamp(gn), wsel);
-            wm_load(Bn, rowi, rowm, wq, clamp(gn + NW), wsel);
-            wm_add(A, hcol, qcol8, sh);
-            if (g + NW < g1) wm_add(B, hcol, qcol8, sh);
+            wm_load(An, rowi, rowm, wq, clamp(gn), half, wsel);
+            wm_load(Bn, rowi, rowm, wq, clamp(gn + NW), half, wsel);
+            wm_add(A, hcol, qcol8);
+            if (g + NW < g1) wm_add(B, hcol, qcol8);
             if (gn >= g1) break;
             A = An;
             B = Bn;
             g = gn;
         }
     }
-    // the last nseq % 16 sequences (weights in natural order), by the lower
-    // half of the first wave of the last part
+    // the last nseq % 32 sequences, by the lower half of the first wave of
+    // the last part
     if (part + 1 == nsplit && wave == 0 && valid && !upper) {
-        for (uint32_t n = ngroups * 16; n < nseq; n++)
+        for (uint32_t n = ngroups * 32; n < nseq; n++)
             atomicAdd(&hist64[__umul24(rowi[n], q * WM_COLS) + rowm[n] * WM_COLS + col],
                       (unsigned long long)wq[n]);
     }
     __syncthreads();
     for (uint32_t e = tid; e < qq * WM_COLS; e += WM_THREADS) {
         const uint32_t c = e & (WM_COLS - 1), ab = e / WM_COLS;
-        const uint32_t mm = cp * WM_COLS + c;
         const unsigned long long v = hist64[e];
-        if (mm > i && mm < L && v)
-            atomicAdd(&acc[(size_t)(pair_index(i, mm, L) - pair_lo) * qq + ab], v);
+        if (p0 + c < npairs && v) atomicAdd(&acc[(size_t)(p0 + c - pair_lo) * qq + ab], v);
     }
 }
 

@@ -348,21 +348,16 @@ int mi3_create(int device, uint32_t L, uint32_t q, uint32_t nwalkers, uint64_t s
         hipEventCreate(&c->t0);
         hipEventRecord(c->t0, c->stream);
     }
-    // weighted-marginal work units: (row i, group cp of 32 partner sites)
-    // with the group holding some m > i; each unit's sequence range is split
+    // weighted-marginal work units: 32 consecutive pairs of the packed order
+    // each (the last one may be short); each unit's sequence range is split
     // in wm_split parts, chosen to fill the last wave of work-groups best
     std::vector<uint32_t> units;
-    const uint32_t ngrp = (L + WM_COLS - 1) / WM_COLS;
-    for (uint32_t i = 0; i + 1 < L; i++)
-        for (uint32_t cp = (i + 1) / WM_COLS; cp < ngrp; cp++) {
-            units.push_back((i << 8) | cp);
-            // pairs (i, m), m in [max(i+1, 32 cp), min(L-1, 32 cp + 31)]
-            const uint32_t m0 = std::max(i + 1, cp * WM_COLS);
-            const uint32_t m1 = std::min(L - 1, cp * WM_COLS + WM_COLS - 1);
-            const uint32_t row0 = i * (2 * L - i - 1) / 2;
-            c->unit_pair_lo.push_back(row0 + (m0 - i - 1));
-            c->unit_pair_hi.push_back(row0 + (m1 - i - 1) + 1);
-        }
+    const uint32_t npairs = L * (L - 1) / 2;
+    for (uint32_t u = 0; u * WM_COLS < npairs; u++) {
+        units.push_back(u);
+        c->unit_pair_lo.push_back(u * WM_COLS);
+        c->unit_pair_hi.push_back(std::min(npairs, (u + 1) * WM_COLS));
+    }
     c->n_units = (uint32_t)units.size();
     {
         double best = 0.0;
