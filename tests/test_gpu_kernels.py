@@ -283,6 +283,32 @@ def test_weighted_marg_ragged_large(gpu_ok, tmp_path):
     assert np.all(F[~occ] == 0)
 
 
+@pytest.mark.parametrize("L", [2, 3, 32, 33, 95])
+def test_weighted_marg_unit_edges(gpu_ok, tmp_path, L):
+    """Work units of 32 consecutive pairs: one pair (L=2, 31 idle columns), a
+    unit spanning every row (L=3), units ending exactly on a row (L=32, 33
+    with 496 and 528 pairs) and the short last rows of L=95; 991 sequences
+    leave a tail of 31 after the last whole group of 32."""
+    q, n = 5, 991
+    g = make_gpu(tmp_path, L, q, 512)
+    g.initJstep()
+    g.initLargeBufs(n)
+    s = seqs_rand(n, L, q, 11)
+    g.setBuf('seq large', s)
+    w = np.random.default_rng(12).exponential(size=n).astype('<f4')
+    g.setBuf('weights large', w)
+    g.weightedMarg('large')
+    F = g.getBuf('bi').read()
+    Fx = cboracle.weighted_marg(s, w, q)
+    occ = Fx > 0
+    np.testing.assert_allclose(F[occ], Fx[occ], rtol=2.4e-7, atol=0)
+    assert np.all(F[~occ] == 0)
+    # unit weights: the counts exactly
+    g.setBuf('weights large', np.ones(n, '<f4'))
+    g.weightedMarg('large')
+    np.testing.assert_array_equal(g.getBuf('bi').read(), cboracle.bicounts(s, q).astype('f4'))
+
+
 def test_min_weights_stats(gpu_ok, tmp_path):
     L, q, N = 20, 4, 4096
     g = make_gpu(tmp_path, L, q, N)
