@@ -17,9 +17,14 @@ constexpr int NT = 64 * NWAVES;
 // L2 -> LDS traffic per sequence.  16 slots in 16 waves (512 sequences per
 // work-group) up to four state words, 8 beyond: with byte gather addresses
 // this beats 32 slots in 8 waves for two state words (2.15 against 2.45 ms
-// at L=232 q=21, 1.31 against 1.67 ms at L=200 q=8)
-template <int KW>
-constexpr int eslots() { return KW <= 4 ? 16 : 8; }
+// at L=232 q=21, 1.31 against 1.67 ms at L=200 q=8).  One state word with the
+// single-pass chunks of q <= 8 takes 8 slots: 45 registers, so two
+// work-groups share a CU and one's barriers overlap the other's gathers
+// (0.33 against 0.37 ms at L=93 q=4, 0.35 against 0.38 at L=93 q=8); with
+// larger chunks the LDS holds one work-group and 8 slots lose (0.73 against
+// 0.49 ms at L=93 q=21)
+template <int KW, int PF>
+constexpr int eslots() { return (KW == 1 && PF == 1) ? 8 : KW <= 4 ? 16 : 8; }
 template <int KW>
 constexpr int ewaves() { return 16; }
 
@@ -49,14 +54,14 @@ hipError_t sweep_kws(const SweepArgs& a, hipStream_t st) {
 template <int KW, int PF>
 hipError_t energies_kw(const EnergyArgs& a, hipStream_t st) {
     constexpr int NW = ewaves<KW>();
-    constexpr uint32_t W = 2 * eslots<KW>() * NW;
+    constexpr uint32_t W = 2 * eslots<KW, PF>() * NW;
     // with four state words (L = 385..512), the 57 KB chunks of q = 17..21
     // stream through the overlapping four-slot ring (all 160 KB of LDS):
     // 7.10 against 7.94 ms at L=400, 10.7 against 11.8 ms at L=500.  The
     // others are double-buffered (the ring took 6% longer at L=300 and 1% at
     // L=232)
     constexpr bool OVL = PF == 7 && KW == 4;
-    auto kern = k_energies<KW, eslots<KW>(), NW, PF, OVL>;
+    auto kern = k_energies<KW, eslots<KW, PF>(), NW, PF, OVL>;
     size_t lds = OVL ? (size_t)OverlapRing::BYTES : 2ull * chunk_floats(a.q) * sizeof(float);
     hipError_t e = set_lds(kern, lds);
     if (e != hipSuccess) return e;
