@@ -315,12 +315,14 @@ MI3_SDWA_BYTE(3)
 
 // hcol: the histogram's base plus this lane's column offset (formed once, so
 // that an address is one add of the row and residue offsets to it)
+template <int W>
+__device__ __forceinline__ uint32_t word(const uint4& v) {
+    return W == 0 ? v.x : W == 1 ? v.y : W == 2 ? v.z : v.w;
+}
 template <int E>
 __device__ __forceinline__ void wm_add1(const WmGroup& G, char* hcol, uint32_t qcol8) {
-    const uint32_t si = (E >> 2) == 0 ? G.si.x : (E >> 2) == 1 ? G.si.y : (E >> 2) == 2 ? G.si.z : G.si.w;
-    const uint32_t sm = (E >> 2) == 0 ? G.sm.x : (E >> 2) == 1 ? G.sm.y : (E >> 2) == 2 ? G.sm.z : G.sm.w;
-    const uint32_t arow = mul_byte<E & 3>(qcol8, si);
-    const uint32_t bcol = byte_x256<E & 3>(sm);
+    const uint32_t arow = mul_byte<E & 3>(qcol8, word<E / 4>(G.si));
+    const uint32_t bcol = byte_x256<E & 3>(word<E / 4>(G.sm));
     atomicAdd(reinterpret_cast<unsigned long long*>(hcol + (arow + bcol)),
               (unsigned long long)row_bcast64(G.w, E));
 }
